@@ -1,0 +1,367 @@
+"""MI355X-native W-projection gridder (drop-in for oskar_grid_wproj_f/_d).
+
+The product is the C-ABI shared library ``libfpga_gridding_amd.so`` (HIP
+kernels for gfx950 behind the reference's C signatures, see
+``include/oskar_grid_wproj.h`` and ``include/fpga_gridding_amd.h``).  This
+module binds it with ctypes so that Python tests and benchmarks call the
+very same entry points a C caller links against.  There is no CPU fallback:
+if the library is missing, or no GPU is present, the calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfpga_gridding_amd.so")
+
+GW_SINGLE = 0x01
+GW_DOUBLE = 0x02
+
+_lib = None
+
+# Every symbol the public headers declare (checked by the CPU tests).
+EXPORTED_SYMBOLS = (
+    "oskar_grid_wproj_f", "oskar_grid_wproj_d",
+    "oskar_grid_wproj_f_diagnostics",
+    "oskar_grid_wproj_f_status", "oskar_grid_wproj_d_status",
+    "oskar_grid_wproj_last_error", "gw_error_string", "gw_last_error_detail",
+    "gw_release_cached",
+    "gw_plan_create", "gw_plan_free", "gw_plan_set_stream", "gw_grid_device",
+    "gw_plan_results", "gw_plan_timings",
+    "gw_device_count", "gw_device_malloc", "gw_device_free",
+    "gw_memcpy_h2d", "gw_memcpy_d2h", "gw_memset_device",
+    "gw_device_synchronize",
+)
+
+
+class GridderError(RuntimeError):
+    """A non-zero status from the library (OSKAR error codes)."""
+
+    def __init__(self, status: int, where: str):
+        L = lib()
+        msg = L.gw_error_string(status).decode()
+        detail = L.gw_last_error_detail().decode()
+        super().__init__(f"{where}: status {status} ({msg}) {detail}".strip())
+        self.status = status
+
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_sz = ct.c_size_t
+_szp = ct.POINTER(ct.c_size_t)
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int)
+_vp = ct.c_void_p
+
+
+def _wproj_args(fp, real):
+    return [_sz, _i32p, ct.c_int, ct.c_int, fp, _sz, fp, fp, fp, fp, _vp,
+            real, real, ct.c_int, _szp, _dp, fp]
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m "
+            "fpga_gridding_amd.build` (there is no CPU fallback)")
+    L = ct.CDLL(LIB_PATH, mode=ct.RTLD_LOCAL)
+    L.oskar_grid_wproj_f_status.argtypes = _wproj_args(_f32p, ct.c_float) + [_ip]
+    L.oskar_grid_wproj_f_status.restype = None
+    L.oskar_grid_wproj_d_status.argtypes = _wproj_args(_f64p, ct.c_double) + [_ip]
+    L.oskar_grid_wproj_d_status.restype = None
+    L.oskar_grid_wproj_f.argtypes = _wproj_args(_f32p, ct.c_float)
+    L.oskar_grid_wproj_f.restype = None
+    L.oskar_grid_wproj_d.argtypes = _wproj_args(_f64p, ct.c_double)
+    L.oskar_grid_wproj_d.restype = None
+    L.oskar_grid_wproj_f_diagnostics.argtypes = _wproj_args(
+        _f32p, ct.c_float) + [_i64p, _i64p, _i64p, _f32p, _vp]
+    L.oskar_grid_wproj_f_diagnostics.restype = None
+    L.oskar_grid_wproj_last_error.argtypes = []
+    L.oskar_grid_wproj_last_error.restype = ct.c_int
+    L.gw_error_string.argtypes = [ct.c_int]
+    L.gw_error_string.restype = ct.c_char_p
+    L.gw_last_error_detail.argtypes = []
+    L.gw_last_error_detail.restype = ct.c_char_p
+    L.gw_release_cached.argtypes = []
+    L.gw_release_cached.restype = None
+    L.gw_plan_create.argtypes = [ct.c_int, _sz, _i32p, ct.c_int, ct.c_int,
+                                 _vp, ct.c_double, ct.c_double, ct.c_int, _ip]
+    L.gw_plan_create.restype = _vp
+    L.gw_plan_free.argtypes = [_vp]
+    L.gw_plan_free.restype = None
+    L.gw_plan_set_stream.argtypes = [_vp, _vp]
+    L.gw_plan_set_stream.restype = None
+    L.gw_grid_device.argtypes = [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _ip]
+    L.gw_grid_device.restype = None
+    L.gw_plan_results.argtypes = [_vp, _szp, _dp,
+                                  ct.POINTER(ct.c_ulonglong), _ip]
+    L.gw_plan_results.restype = None
+    L.gw_plan_timings.argtypes = [_vp, _dp, ct.c_int]
+    L.gw_plan_timings.restype = ct.c_int
+    L.gw_device_count.argtypes = []
+    L.gw_device_count.restype = ct.c_int
+    L.gw_device_malloc.argtypes = [_sz, _ip]
+    L.gw_device_malloc.restype = _vp
+    L.gw_device_free.argtypes = [_vp]
+    L.gw_device_free.restype = None
+    for name in ("gw_memcpy_h2d", "gw_memcpy_d2h"):
+        getattr(L, name).argtypes = [_vp, _vp, _sz, _ip]
+        getattr(L, name).restype = None
+    L.gw_memset_device.argtypes = [_vp, ct.c_int, _sz, _ip]
+    L.gw_memset_device.restype = None
+    L.gw_device_synchronize.argtypes = [_ip]
+    L.gw_device_synchronize.restype = None
+    _lib = L
+    return L
+
+
+def _check(status: ct.c_int, where: str):
+    if status.value != 0:
+        raise GridderError(status.value, where)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# --------------------------------------------------------------------------
+# Drop-in calls (host arrays), same argument meaning as the reference
+# oskar_grid_wproj_f / _d (oskar_grid_wproj.h:50-66).
+# --------------------------------------------------------------------------
+
+def grid_wproj_f(num_w_planes, support, oversample, conv_size_half, conv_func,
+                 uu, vv, ww, vis, weight, cell_size_rad, w_scale, grid_size,
+                 grid, norm=0.0):
+    """FP32 gridding into ``grid`` (float32 [2*G*G], accumulated in place).
+
+    Returns ``(num_skipped, norm)`` with ``norm`` the updated value."""
+    if grid.dtype != np.float32 or not grid.flags.c_contiguous:
+        raise TypeError("grid must be a C-contiguous float32 array")
+    if grid.size < 2 * grid_size * grid_size:
+        raise ValueError("grid is smaller than 2*grid_size^2")
+    uu, vv, ww = _c(uu, np.float32), _c(vv, np.float32), _c(ww, np.float32)
+    vis = _c(vis, np.float32)
+    n = uu.shape[0]
+    if vv.shape[0] != n or ww.shape[0] != n or vis.shape[0] != 2 * n:
+        raise ValueError("uu, vv, ww need N values and vis 2N")
+    sk = ct.c_size_t(0)
+    nrm = ct.c_double(norm)
+    st = ct.c_int(0)
+    lib().oskar_grid_wproj_f_status(
+        num_w_planes, _c(support, np.int32), oversample, conv_size_half,
+        _c(conv_func, np.float32), n, uu, vv, ww, vis, None,
+        cell_size_rad, w_scale, grid_size, ct.byref(sk), ct.byref(nrm), grid,
+        ct.byref(st))
+    _check(st, "oskar_grid_wproj_f")
+    return int(sk.value), float(nrm.value)
+
+
+def grid_wproj_d(num_w_planes, support, oversample, conv_size_half, conv_func,
+                 uu, vv, ww, vis, weight, cell_size_rad, w_scale, grid_size,
+                 grid, norm=0.0):
+    """FP64 gridding into ``grid`` (float64 [2*G*G], accumulated)."""
+    if grid.dtype != np.float64 or not grid.flags.c_contiguous:
+        raise TypeError("grid must be a C-contiguous float64 array")
+    if grid.size < 2 * grid_size * grid_size:
+        raise ValueError("grid is smaller than 2*grid_size^2")
+    uu, vv, ww = _c(uu, np.float64), _c(vv, np.float64), _c(ww, np.float64)
+    vis = _c(vis, np.float64)
+    n = uu.shape[0]
+    if vv.shape[0] != n or ww.shape[0] != n or vis.shape[0] != 2 * n:
+        raise ValueError("uu, vv, ww need N values and vis 2N")
+    sk = ct.c_size_t(0)
+    nrm = ct.c_double(norm)
+    st = ct.c_int(0)
+    lib().oskar_grid_wproj_d_status(
+        num_w_planes, _c(support, np.int32), oversample, conv_size_half,
+        _c(conv_func, np.float64), n, uu, vv, ww, vis, None,
+        cell_size_rad, w_scale, grid_size, ct.byref(sk), ct.byref(nrm), grid,
+        ct.byref(st))
+    _check(st, "oskar_grid_wproj_d")
+    return int(sk.value), float(nrm.value)
+
+
+def grid_wproj_f_diagnostics(num_w_planes, support, oversample,
+                             conv_size_half, conv_func, uu, vv, ww, vis,
+                             weight, cell_size_rad, w_scale, grid_size, grid,
+                             per_support, per_plane, per_cell, avg_support,
+                             norm=0.0):
+    """oskar_grid_wproj_f_diagnostics (oskar_grid_wproj.c:245-370)."""
+    uu, vv, ww = _c(uu, np.float32), _c(vv, np.float32), _c(ww, np.float32)
+    vis = _c(vis, np.float32)
+    n = uu.shape[0]
+    sk = ct.c_size_t(0)
+    nrm = ct.c_double(norm)
+    L = lib()
+    L.oskar_grid_wproj_f_diagnostics(
+        num_w_planes, _c(support, np.int32), oversample, conv_size_half,
+        _c(conv_func, np.float32), n, uu, vv, ww, vis, None,
+        cell_size_rad, w_scale, grid_size, ct.byref(sk), ct.byref(nrm), grid,
+        per_support, per_plane, per_cell, avg_support, None)
+    st = L.oskar_grid_wproj_last_error()
+    if st:
+        raise GridderError(st, "oskar_grid_wproj_f_diagnostics")
+    return int(sk.value), float(nrm.value)
+
+
+def grid_workload_f(wl, grid=None, norm=0.0):
+    """Drop-in FP32 call on a synth.Workload; returns (grid, skipped, norm)."""
+    g = wl.grid_size
+    if grid is None:
+        grid = np.zeros(2 * g * g, dtype=np.float32)
+    sk, nrm = grid_wproj_f(wl.num_w_planes, wl.support, wl.oversample,
+                           wl.conv_size_half, wl.conv_func, wl.uu, wl.vv,
+                           wl.ww, wl.vis, wl.weight, wl.cell_size_rad,
+                           wl.w_scale, g, grid, norm)
+    return grid, sk, nrm
+
+
+def grid_workload_d(wl, grid=None, norm=0.0):
+    """Drop-in FP64 call on a synth.Workload (inputs widened)."""
+    g = wl.grid_size
+    if grid is None:
+        grid = np.zeros(2 * g * g, dtype=np.float64)
+    sk, nrm = grid_wproj_d(wl.num_w_planes, wl.support, wl.oversample,
+                           wl.conv_size_half, wl.conv_func.astype(np.float64),
+                           wl.uu, wl.vv, wl.ww, wl.vis.astype(np.float64),
+                           None, float(np.float32(wl.cell_size_rad)),
+                           float(np.float32(wl.w_scale)), g, grid, norm)
+    return grid, sk, nrm
+
+
+# --------------------------------------------------------------------------
+# Device buffers and the resident plan
+# --------------------------------------------------------------------------
+
+class DeviceArray:
+    """A device allocation owned by Python (freed on close/GC)."""
+
+    def __init__(self, nbytes: int):
+        st = ct.c_int(0)
+        self.ptr = lib().gw_device_malloc(nbytes, ct.byref(st))
+        _check(st, "gw_device_malloc")
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_host(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("host array larger than the device buffer")
+        st = ct.c_int(0)
+        lib().gw_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes, ct.byref(st))
+        _check(st, "gw_memcpy_h2d")
+
+    def download(self, out: np.ndarray):
+        if out.nbytes > self.nbytes or not out.flags.c_contiguous:
+            raise ValueError("bad output array")
+        st = ct.c_int(0)
+        lib().gw_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes,
+                            ct.byref(st))
+        _check(st, "gw_memcpy_d2h")
+        return out
+
+    def zero(self):
+        st = ct.c_int(0)
+        lib().gw_memset_device(self.ptr, 0, self.nbytes, ct.byref(st))
+        _check(st, "gw_memset_device")
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().gw_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    st = ct.c_int(0)
+    lib().gw_device_synchronize(ct.byref(st))
+    _check(st, "gw_device_synchronize")
+
+
+def device_count() -> int:
+    return lib().gw_device_count()
+
+
+class Plan:
+    """Resident plan: W-kernel cube and tables on the device; grids
+    device-resident points into a device-resident grid."""
+
+    def __init__(self, precision, num_w_planes, support, oversample,
+                 conv_size_half, conv_func, cell_size_rad, w_scale,
+                 grid_size):
+        dt = np.float32 if precision == GW_SINGLE else np.float64
+        cube = _c(conv_func, dt)
+        st = ct.c_int(0)
+        self._cube = cube
+        self.precision = precision
+        self.grid_size = grid_size
+        self.ptr = lib().gw_plan_create(
+            precision, num_w_planes, _c(support, np.int32), oversample,
+            conv_size_half, cube.ctypes.data, cell_size_rad, w_scale,
+            grid_size, ct.byref(st))
+        _check(st, "gw_plan_create")
+
+    @classmethod
+    def for_workload(cls, wl, precision=GW_SINGLE):
+        return cls(precision, wl.num_w_planes, wl.support, wl.oversample,
+                   wl.conv_size_half, wl.conv_func, wl.cell_size_rad,
+                   wl.w_scale, wl.grid_size)
+
+    def grid(self, n, d_uu, d_vv, d_ww, d_vis, d_grid):
+        st = ct.c_int(0)
+        lib().gw_grid_device(self.ptr, n, _ptr(d_uu), _ptr(d_vv), _ptr(d_ww),
+                             _ptr(d_vis), _ptr(d_grid), ct.byref(st))
+        _check(st, "gw_grid_device")
+
+    def results(self):
+        st = ct.c_int(0)
+        sk = ct.c_size_t(0)
+        nd = ct.c_double(0)
+        nu = ct.c_ulonglong(0)
+        lib().gw_plan_results(self.ptr, ct.byref(sk), ct.byref(nd),
+                              ct.byref(nu), ct.byref(st))
+        _check(st, "gw_plan_results")
+        return int(sk.value), float(nd.value), int(nu.value)
+
+    def timings(self):
+        buf = (ct.c_double * 8)()
+        n = lib().gw_plan_timings(self.ptr, buf, 8)
+        return list(buf[:n])
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().gw_plan_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptr(x):
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    return x

@@ -1,0 +1,596 @@
+// C ABI of the MI355X W-projection gridder.
+//
+// Drop-in entry points with the reference's signatures
+// (oskar_grid_wproj.h:68-131, oskar_grid_wproj.c:245-267), OSKAR-style
+// status variants, the resident plan API and device-memory helpers.  The
+// drop-in calls take HOST pointers: each call uploads the points, runs the
+// device pipeline of gridder.hip, and moves only the grid rows that the
+// footprints touch across PCIe (read-modify-write of the caller's
+// accumulating grid).  A per-thread cache keeps the plan (device W-kernel
+// cube and tables) and the device buffers between calls with the same
+// kernel set, so repeated calls per visibility block (main.cpp:413-535)
+// upload the cube once.
+#include "gridder_internal.h"
+
+#include "fpga_gridding_amd.h"
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local int t_last_error = 0;
+thread_local std::string t_last_detail;
+
+void set_error(int code, const std::string& detail)
+{
+    t_last_error = code;
+    t_last_detail = detail;
+}
+
+// Device buffer that only grows.
+struct DevBuf
+{
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void* get(size_t need)
+    {
+        if (need > bytes)
+        {
+            if (ptr) (void)hipFree(ptr);
+            ptr = nullptr;
+            bytes = 0;
+            gw::check_hip(hipMalloc(&ptr, std::max<size_t>(need, 256)),
+                    gw::kErrMemAlloc, "hipMalloc");
+            bytes = std::max<size_t>(need, 256);
+        }
+        return ptr;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+// Per-thread state of the drop-in calls.  Deliberately never destroyed at
+// thread or process exit (the HIP runtime may be gone by then); released
+// explicitly by gw_release_cached().
+struct DropinCache
+{
+    std::unique_ptr<gw::Plan> plan;
+    int prec = 0;
+    size_t P = 0;
+    std::vector<int> support;
+    int ov = 0, C = 0, G = 0;
+    double cell = 0, ws = 0;
+    std::vector<char> cube;
+    int device = -1;
+    DevBuf in, grid, d_support_hist, d_plane_hist, d_cell_count, d_cell_wsum;
+
+    bool matches(int prec_, size_t P_, const int* support_, int ov_, int C_,
+                 const void* conv, size_t cube_bytes, double cell_,
+                 double ws_, int G_, int dev) const
+    {
+        return plan && prec == prec_ && P == P_ && ov == ov_ && C == C_ &&
+                G == G_ && cell == cell_ && ws == ws_ && device == dev &&
+                cube.size() == cube_bytes &&
+                std::equal(support.begin(), support.end(), support_) &&
+                std::memcmp(cube.data(), conv, cube_bytes) == 0;
+    }
+};
+
+thread_local DropinCache* t_cache = nullptr;
+
+DropinCache& cache()
+{
+    if (!t_cache) t_cache = new DropinCache();
+    return *t_cache;
+}
+
+// Caller-side diagnostics arrays of oskar_grid_wproj_f_diagnostics.
+struct DiagHost
+{
+    long long* per_support;
+    long long* per_plane;
+    long long* per_cell;
+    float* wsum_cell;
+};
+
+template <typename T>
+void run_dropin(int prec, size_t P, const int* support, int ov, int C,
+                const T* conv, size_t n, const T* uu, const T* vv,
+                const T* ww, const T* vis, T cell, T ws, int G,
+                size_t* num_skipped, double* norm, T* grid,
+                const DiagHost* dh)
+{
+    if (!num_skipped || !norm || !grid)
+        gw::fail(gw::kErrInvalidArgument,
+                "num_skipped, norm and grid must not be NULL");
+    if (n > 0 && (!uu || !vv || !ww || !vis))
+        gw::fail(gw::kErrInvalidArgument, "uu, vv, ww and vis must not be NULL");
+    if (dh && (!dh->per_support || !dh->per_plane || !dh->per_cell ||
+            !dh->wsum_cell))
+        gw::fail(gw::kErrInvalidArgument, "diagnostics arrays must not be NULL");
+    if (!support || !conv)
+        gw::fail(gw::kErrInvalidArgument, "support and conv_func must not be NULL");
+    if (P < 1 || C < 1)
+        gw::fail(gw::kErrInvalidArgument, "num_w_planes and conv_size_half must be >= 1");
+
+    int dev = 0;
+    gw::check_hip(hipGetDevice(&dev), gw::kErrComputeDevices, "hipGetDevice");
+    DropinCache& c = cache();
+    const size_t cube_bytes = 2 * P * (size_t)C * C * sizeof(T);
+    if (!c.matches(prec, P, support, ov, C, conv, cube_bytes, (double)cell,
+            (double)ws, G, dev))
+    {
+        c.plan.reset();
+        c.plan.reset(new gw::Plan(prec, P, support, ov, C, conv, (double)cell,
+                (double)ws, G));
+        c.prec = prec;
+        c.P = P;
+        c.support.assign(support, support + P);
+        c.ov = ov;
+        c.C = C;
+        c.G = G;
+        c.cell = (double)cell;
+        c.ws = (double)ws;
+        c.device = dev;
+        c.cube.assign((const char*)conv, (const char*)conv + cube_bytes);
+    }
+    gw::Plan& plan = *c.plan;
+    hipStream_t s = plan.stream();
+
+    if (n == 0)
+    {
+        // The reference resets the count and grids nothing (:153-155).
+        *num_skipped = 0;
+        return;
+    }
+
+    // Inputs: uu | vv | ww | vis, one device allocation.
+    T* d_in = (T*)c.in.get(5 * n * sizeof(T));
+    gw::check_hip(hipMemcpyAsync(d_in, uu, n * sizeof(T),
+            hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload uu");
+    gw::check_hip(hipMemcpyAsync(d_in + n, vv, n * sizeof(T),
+            hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload vv");
+    gw::check_hip(hipMemcpyAsync(d_in + 2 * n, ww, n * sizeof(T),
+            hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload ww");
+    gw::check_hip(hipMemcpyAsync(d_in + 3 * n, vis, 2 * n * sizeof(T),
+            hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload vis");
+
+    gw::DiagBuffers diag;
+    if (dh)
+    {
+        diag.max_support = plan.max_support();
+        diag.plane_cap = (int)P + 1024;
+        diag.per_support = (unsigned long long*)c.d_support_hist.get(
+                (diag.max_support + 1) * sizeof(unsigned long long));
+        diag.per_plane = (unsigned long long*)c.d_plane_hist.get(
+                diag.plane_cap * sizeof(unsigned long long));
+        gw::fill_zero_u64(diag.per_support, diag.max_support + 1, s);
+        gw::fill_zero_u64(diag.per_plane, diag.plane_cap, s);
+    }
+
+    plan.locate(n, d_in, d_in + n, d_in + 2 * n, d_in + 3 * n,
+            dh ? &diag : nullptr);
+    gw::Summary sm = plan.read_summary();
+    if (dh && (sm.flags & 1u))
+        gw::fail(gw::kErrOutOfRange,
+                "diagnostics: a plane index exceeds num_w_planes + 1023");
+
+    const bool any = sm.skipped < n;
+    int row0 = 0, rows = 0;
+    if (any)
+    {
+        row0 = sm.v_min;
+        rows = sm.v_max - sm.v_min + 1;
+        const size_t row_vals = 2 * (size_t)G;
+        T* d_grid = (T*)c.grid.get((size_t)rows * row_vals * sizeof(T));
+        gw::check_hip(hipMemcpyAsync(d_grid, grid + (size_t)row0 * row_vals,
+                (size_t)rows * row_vals * sizeof(T), hipMemcpyHostToDevice,
+                s), gw::kErrMemCopy, "upload grid rows");
+        if (dh)
+        {
+            const size_t cells = (size_t)rows * G;
+            diag.per_cell = (unsigned*)c.d_cell_count.get(
+                    cells * sizeof(unsigned));
+            diag.wsum_cell = (unsigned long long*)c.d_cell_wsum.get(
+                    cells * sizeof(unsigned long long));
+            gw::fill_zero_u32(diag.per_cell, cells, s);
+            gw::fill_zero_u64(diag.wsum_cell, cells, s);
+        }
+        plan.scatter(n, d_grid, row0, dh ? &diag : nullptr);
+        sm = plan.read_summary();
+        if (sm.flags & 2u)
+            gw::fail(gw::kErrOutOfRange, "internal: work item table overflow");
+        gw::check_hip(hipMemcpyAsync(grid + (size_t)row0 * row_vals, d_grid,
+                (size_t)rows * row_vals * sizeof(T), hipMemcpyDeviceToHost,
+                s), gw::kErrMemCopy, "download grid rows");
+    }
+    if (dh)
+    {
+        std::vector<unsigned long long> hs(diag.max_support + 1),
+                hp(diag.plane_cap);
+        gw::check_hip(hipMemcpyAsync(hs.data(), diag.per_support,
+                hs.size() * sizeof(unsigned long long),
+                hipMemcpyDeviceToHost, s), gw::kErrMemCopy, "download hist");
+        gw::check_hip(hipMemcpyAsync(hp.data(), diag.per_plane,
+                hp.size() * sizeof(unsigned long long),
+                hipMemcpyDeviceToHost, s), gw::kErrMemCopy, "download hist");
+        std::vector<unsigned> hc;
+        std::vector<unsigned long long> hw;
+        if (any)
+        {
+            const size_t cells = (size_t)rows * G;
+            hc.resize(cells);
+            hw.resize(cells);
+            gw::check_hip(hipMemcpyAsync(hc.data(), diag.per_cell,
+                    cells * sizeof(unsigned), hipMemcpyDeviceToHost, s),
+                    gw::kErrMemCopy, "download cell counts");
+            gw::check_hip(hipMemcpyAsync(hw.data(), diag.wsum_cell,
+                    cells * sizeof(unsigned long long),
+                    hipMemcpyDeviceToHost, s), gw::kErrMemCopy,
+                    "download cell sums");
+        }
+        gw::check_hip(hipStreamSynchronize(s), gw::kErrKernelLaunch,
+                "stream synchronize");
+        // Touch only the entries the reference would touch.
+        for (size_t i = 0; i < hs.size(); ++i)
+            if (hs[i]) dh->per_support[i] += (long long)hs[i];
+        for (size_t i = 0; i < hp.size(); ++i)
+            if (hp[i]) dh->per_plane[i] += (long long)hp[i];
+        for (size_t i = 0; i < hc.size(); ++i)
+        {
+            if (!hc[i]) continue;
+            const size_t cell_i = (size_t)row0 * G + i;
+            dh->per_cell[cell_i] += (long long)hc[i];
+            dh->wsum_cell[cell_i] += (float)hw[i];
+        }
+    }
+    else
+    {
+        gw::check_hip(hipStreamSynchronize(s), gw::kErrKernelLaunch,
+                "stream synchronize");
+    }
+    *num_skipped = (size_t)sm.skipped;
+    *norm += sm.norm_delta;
+}
+
+template <typename F>
+void guarded(int* status, F&& f)
+{
+    try
+    {
+        f();
+    }
+    catch (const gw::Error& e)
+    {
+        *status = e.code;
+        set_error(e.code, e.detail);
+    }
+    catch (const std::bad_alloc&)
+    {
+        *status = gw::kErrMemAlloc;
+        set_error(gw::kErrMemAlloc, "host allocation failed");
+    }
+    catch (...)
+    {
+        *status = gw::kErrFunctionNotAvailable;
+        set_error(gw::kErrFunctionNotAvailable, "unexpected exception");
+    }
+}
+
+}  // namespace
+
+struct gw_plan
+{
+    gw::Plan* impl;
+};
+
+extern "C" {
+
+GW_EXPORT
+void oskar_grid_wproj_f_status(const size_t num_w_planes, const int* support,
+        const int oversample, const int conv_size_half,
+        const float* conv_func, const size_t num_points, const float* uu,
+        const float* vv, const float* ww, const float* vis,
+        const float* weight, const float cell_size_rad, const float w_scale,
+        const int grid_size, size_t* num_skipped, double* norm, float* grid,
+        int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    (void)weight;
+    guarded(status, [&] {
+        run_dropin<float>(GW_SINGLE, num_w_planes, support, oversample,
+                conv_size_half, conv_func, num_points, uu, vv, ww, vis,
+                cell_size_rad, w_scale, grid_size, num_skipped, norm, grid,
+                nullptr);
+    });
+}
+
+GW_EXPORT
+void oskar_grid_wproj_d_status(const size_t num_w_planes, const int* support,
+        const int oversample, const int conv_size_half,
+        const double* conv_func, const size_t num_points, const double* uu,
+        const double* vv, const double* ww, const double* vis,
+        const double* weight, const double cell_size_rad,
+        const double w_scale, const int grid_size, size_t* num_skipped,
+        double* norm, double* grid, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    (void)weight;
+    guarded(status, [&] {
+        run_dropin<double>(GW_DOUBLE, num_w_planes, support, oversample,
+                conv_size_half, conv_func, num_points, uu, vv, ww, vis,
+                cell_size_rad, w_scale, grid_size, num_skipped, norm, grid,
+                nullptr);
+    });
+}
+
+GW_EXPORT
+void oskar_grid_wproj_f(const size_t num_w_planes, const int* support,
+        const int oversample, const int conv_size_half,
+        const float* conv_func, const size_t num_points, const float* uu,
+        const float* vv, const float* ww, const float* vis,
+        const float* weight, const float cell_size_rad, const float w_scale,
+        const int grid_size, size_t* num_skipped, double* norm, float* grid)
+{
+    int status = 0;
+    t_last_error = 0;
+    t_last_detail.clear();
+    oskar_grid_wproj_f_status(num_w_planes, support, oversample,
+            conv_size_half, conv_func, num_points, uu, vv, ww, vis, weight,
+            cell_size_rad, w_scale, grid_size, num_skipped, norm, grid,
+            &status);
+    t_last_error = status;
+}
+
+GW_EXPORT
+void oskar_grid_wproj_d(const size_t num_w_planes, const int* support,
+        const int oversample, const int conv_size_half,
+        const double* conv_func, const size_t num_points, const double* uu,
+        const double* vv, const double* ww, const double* vis,
+        const double* weight, const double cell_size_rad,
+        const double w_scale, const int grid_size, size_t* num_skipped,
+        double* norm, double* grid)
+{
+    int status = 0;
+    t_last_error = 0;
+    t_last_detail.clear();
+    oskar_grid_wproj_d_status(num_w_planes, support, oversample,
+            conv_size_half, conv_func, num_points, uu, vv, ww, vis, weight,
+            cell_size_rad, w_scale, grid_size, num_skipped, norm, grid,
+            &status);
+    t_last_error = status;
+}
+
+GW_EXPORT
+void oskar_grid_wproj_f_diagnostics(const size_t num_w_planes,
+        const int* support, const int oversample, const int conv_size_half,
+        const float* conv_func, const size_t num_points, const float* uu,
+        const float* vv, const float* ww, const float* vis,
+        const float* weight, const float cell_size_rad, const float w_scale,
+        const int grid_size, size_t* num_skipped, double* norm, float* grid,
+        long long int* num_vis_per_wsupport, long long int* num_vis_per_wplane,
+        long long int* num_vis_per_grid, float* avg_wsupport_per_grid,
+        float* avg_wplane_per_grid)
+{
+    int status = 0;
+    t_last_error = 0;
+    t_last_detail.clear();
+    (void)weight;
+    (void)avg_wplane_per_grid;
+    DiagHost dh{num_vis_per_wsupport, num_vis_per_wplane, num_vis_per_grid,
+                avg_wsupport_per_grid};
+    guarded(&status, [&] {
+        run_dropin<float>(GW_SINGLE, num_w_planes, support, oversample,
+                conv_size_half, conv_func, num_points, uu, vv, ww, vis,
+                cell_size_rad, w_scale, grid_size, num_skipped, norm, grid,
+                &dh);
+    });
+    t_last_error = status;
+}
+
+GW_EXPORT int oskar_grid_wproj_last_error(void)
+{
+    return t_last_error;
+}
+
+GW_EXPORT const char* gw_last_error_detail(void)
+{
+    return t_last_detail.c_str();
+}
+
+GW_EXPORT const char* gw_error_string(int status)
+{
+    switch (status)
+    {
+    case GW_OK: return "no error";
+    case GW_ERR_INVALID_ARGUMENT: return "invalid function argument";
+    case GW_ERR_FUNCTION_NOT_AVAILABLE: return "function not available";
+    case GW_ERR_OUT_OF_RANGE: return "value out of range";
+    case GW_ERR_MEMORY_ALLOC_FAILURE: return "memory allocation failure";
+    case GW_ERR_MEMORY_COPY_FAILURE: return "memory copy failure";
+    case GW_ERR_KERNEL_LAUNCH_FAILURE: return "kernel launch failure";
+    case GW_ERR_COMPUTE_DEVICES: return "compute device error";
+    default: return "unknown error";
+    }
+}
+
+GW_EXPORT void gw_release_cached(void)
+{
+    if (t_cache)
+    {
+        t_cache->plan.reset();
+        t_cache->in.release();
+        t_cache->grid.release();
+        t_cache->d_support_hist.release();
+        t_cache->d_plane_hist.release();
+        t_cache->d_cell_count.release();
+        t_cache->d_cell_wsum.release();
+        delete t_cache;
+        t_cache = nullptr;
+    }
+}
+
+GW_EXPORT
+gw_plan* gw_plan_create(int precision, size_t num_w_planes,
+        const int* support, int oversample, int conv_size_half,
+        const void* conv_func, double cell_size_rad, double w_scale,
+        int grid_size, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return nullptr;
+    gw_plan* out = nullptr;
+    guarded(status, [&] {
+        std::unique_ptr<gw::Plan> p(new gw::Plan(precision, num_w_planes,
+                support, oversample, conv_size_half, conv_func,
+                cell_size_rad, w_scale, grid_size));
+        out = new gw_plan{p.release()};
+    });
+    return out;
+}
+
+GW_EXPORT void gw_plan_free(gw_plan* plan)
+{
+    if (!plan) return;
+    delete plan->impl;
+    delete plan;
+}
+
+GW_EXPORT void gw_plan_set_stream(gw_plan* plan, void* hip_stream)
+{
+    if (!plan) return;
+    int status = 0;
+    guarded(&status, [&] {
+        plan->impl->set_stream(static_cast<hipStream_t>(hip_stream));
+    });
+}
+
+GW_EXPORT
+void gw_grid_device(gw_plan* plan, size_t num_points, const void* d_uu,
+        const void* d_vv, const void* d_ww, const void* d_vis, void* d_grid,
+        int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!plan) gw::fail(gw::kErrInvalidArgument, "plan is NULL");
+        if (!d_grid || (num_points && (!d_uu || !d_vv || !d_ww || !d_vis)))
+            gw::fail(gw::kErrInvalidArgument, "device pointers must not be NULL");
+        plan->impl->locate(num_points, d_uu, d_vv, d_ww, d_vis, nullptr);
+        plan->impl->scatter(num_points, d_grid, 0, nullptr);
+    });
+}
+
+GW_EXPORT
+void gw_plan_results(gw_plan* plan, size_t* num_skipped, double* norm_delta,
+        unsigned long long* num_updates, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!plan) gw::fail(gw::kErrInvalidArgument, "plan is NULL");
+        const gw::Summary& sm = plan->impl->read_summary();
+        if (sm.flags & 2u)
+            gw::fail(gw::kErrOutOfRange, "internal: work item table overflow");
+        if (num_skipped) *num_skipped = (size_t)sm.skipped;
+        if (norm_delta) *norm_delta = sm.norm_delta;
+        if (num_updates) *num_updates = sm.updates;
+    });
+}
+
+GW_EXPORT int gw_plan_timings(gw_plan* plan, double* ms, int max_n)
+{
+    if (!plan || !ms) return 0;
+    int n = 0, status = 0;
+    guarded(&status, [&] { n = plan->impl->timings(ms, max_n); });
+    return status ? 0 : n;
+}
+
+GW_EXPORT int gw_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+GW_EXPORT void* gw_device_malloc(size_t bytes, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return nullptr;
+    void* p = nullptr;
+    guarded(status, [&] {
+        gw::check_hip(hipMalloc(&p, bytes ? bytes : 1), gw::kErrMemAlloc,
+                "hipMalloc");
+    });
+    return p;
+}
+
+GW_EXPORT void gw_device_free(void* ptr)
+{
+    if (ptr) (void)hipFree(ptr);
+}
+
+GW_EXPORT void gw_memcpy_h2d(void* dst, const void* src, size_t bytes,
+        int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        gw::check_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice),
+                gw::kErrMemCopy, "hipMemcpy H2D");
+    });
+}
+
+GW_EXPORT void gw_memcpy_d2h(void* dst, const void* src, size_t bytes,
+        int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        gw::check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost),
+                gw::kErrMemCopy, "hipMemcpy D2H");
+    });
+}
+
+GW_EXPORT void gw_memset_device(void* ptr, int value, size_t bytes,
+        int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        gw::check_hip(hipMemset(ptr, value, bytes), gw::kErrMemCopy,
+                "hipMemset");
+    });
+}
+
+GW_EXPORT void gw_device_synchronize(int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        gw::check_hip(hipDeviceSynchronize(), gw::kErrKernelLaunch,
+                "hipDeviceSynchronize");
+    });
+}
+
+}  // extern "C"

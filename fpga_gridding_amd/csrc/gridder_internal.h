@@ -1,0 +1,181 @@
+// Internal declarations shared by the gridder's device code (gridder.hip)
+// and its C ABI layer (capi.cpp).  Not installed; the public interface is
+// include/oskar_grid_wproj.h and include/fpga_gridding_amd.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace gw {
+
+// Largest number of footprint cells one lane owns (registers per slot:
+// two accumulators, the owned grid cell, two tap offsets).
+constexpr int kSlotsMax = 8;
+// Target grid updates per wave work item (load-balance granularity).
+constexpr int kTapsPerItem = 1 << 16;
+// Sentinel for "lane slot owns no grid cell yet".
+constexpr unsigned kNoCell = 0xFFFFFFFFu;
+
+// Per-plane constants, fixed by the support table (host-computed at plan
+// creation, uploaded once).
+struct PlaneInfo
+{
+    int W;       // footprint side 2S+1
+    int S;       // support
+    int nslot;   // cell slots per lane in a wave work item (<= kSlotsMax)
+    int cpw;     // footprint cells per wave work item
+    int L;       // points per chunk
+    int nw;      // wave work items per chunk (ceil(W^2 / cpw))
+    int pad0, pad1;
+};
+
+// Device-side per-call summary (written by the kernels, read back once).
+struct Summary
+{
+    unsigned long long skipped;     // points failing the edge test
+    unsigned long long n_valid;     // points gridded
+    unsigned long long updates;     // sum over gridded points of W^2
+    double norm_delta;              // sum over gridded points of tap sums
+    int v_min, v_max;               // grid rows touched (inclusive)
+    int u_min, u_max;               // grid columns touched (inclusive)
+    unsigned total_items;           // wave work items of the scatter
+    unsigned flags;                 // bit 0: diagnostics plane index beyond
+                                    // its table; bit 1: work items overflow
+};
+
+// Per-point record written by the locate kernel in input order.
+struct alignas(16) RecF { int pos; int sub; float re; float im; };
+struct alignas(16) RecD { int pos; int sub; int pad0, pad1; double re, im; };
+
+template <typename T> struct Types;
+template <> struct Types<float>
+{
+    typedef float2 T2; typedef float4 T4; typedef RecF Rec;
+};
+template <> struct Types<double>
+{
+    typedef double2 T2; typedef double4 T4; typedef RecD Rec;
+};
+
+// Status codes (values of utils/oskar_global.h:46-64).
+enum
+{
+    kOk = 0,
+    kErrInvalidArgument = -3,
+    kErrFunctionNotAvailable = -4,
+    kErrOutOfRange = -5,
+    kErrMemAlloc = -6,
+    kErrMemCopy = -7,
+    kErrKernelLaunch = -18,
+    kErrComputeDevices = -19
+};
+
+// Error raised inside the library and turned into a status code at the C
+// ABI boundary.
+struct Error
+{
+    int code;
+    std::string detail;
+};
+
+[[noreturn]] void fail(int code, const std::string& detail);
+void check_hip(hipError_t e, int code, const char* what);
+
+// Diagnostics outputs (device buffers) of one call.
+struct DiagBuffers
+{
+    unsigned long long* per_support = nullptr;   // [max_support + 1]
+    unsigned long long* per_plane = nullptr;     // [plane_cap]
+    unsigned* per_cell = nullptr;                // [rows * G]
+    unsigned long long* wsum_cell = nullptr;     // [rows * G]
+    int max_support = 0;
+    int plane_cap = 0;
+};
+
+// A resident gridding plan: device W-kernel cube, derived tables, workspace.
+class Plan
+{
+public:
+    Plan(int precision, size_t num_w_planes, const int* support,
+         int oversample, int conv_size_half, const void* conv_func,
+         double cell_size_rad, double w_scale, int grid_size);
+    ~Plan();
+
+    // Stage 1 (enqueue): index math, skip count, histograms, sort keys.
+    void locate(size_t n, const void* uu, const void* vv, const void* ww,
+                const void* vis, const DiagBuffers* diag);
+    // Stages 2-4 (enqueue): sort, gather + plan, scatter into the grid
+    // whose first row held in memory is row0.
+    void scatter(size_t n, void* grid, int row0, const DiagBuffers* diag);
+    // Copy the device summary to the host (synchronises the stream).
+    const Summary& read_summary();
+    void set_stream(hipStream_t s);
+    hipStream_t stream() const { return stream_; }
+    int precision() const { return prec_; }
+    int grid_size() const { return G_; }
+    size_t num_w_planes() const { return P_; }
+    int max_support() const { return s_max_; }
+    int timings(double* ms, int max_n);
+
+private:
+    void ensure_capacity(size_t n);
+    void free_workspace();
+    template <typename T> void locate_t(size_t n, const T* uu, const T* vv,
+            const T* ww, const T* vis, const DiagBuffers* diag);
+    template <typename T> void scatter_t(size_t n, T* grid, int row0,
+            const DiagBuffers* diag);
+
+    int prec_;
+    size_t P_;
+    int ov_, C_, G_, h_, K_;
+    int s_max_;
+    float cell_f_, ws_f_, gscale_f_;
+    double cell_d_, ws_d_, gscale_d_;
+    int coord_bits_;
+    int key_bits_;
+    std::vector<int> support_;
+    std::vector<PlaneInfo> pinfo_;
+    double items_per_point_;   // bound on wave items per gridded point
+    size_t items_fixed_;       // plus this many
+    int device_ = 0;
+    int n_cu_ = 0;
+    int grid_blocks_ = 0;
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+
+    // Constant device tables.
+    void* d_cube_ = nullptr;
+    int* d_support_ = nullptr;
+    PlaneInfo* d_pinfo_ = nullptr;
+    double* d_normtab_ = nullptr;
+
+    // Workspace (grown on demand).
+    size_t cap_ = 0;
+    size_t items_cap_ = 0;
+    unsigned long long* d_keys_[2] = {nullptr, nullptr};
+    unsigned* d_idx_[2] = {nullptr, nullptr};
+    void* d_rec_ = nullptr;
+    int4* d_meta_ = nullptr;
+    void* d_vq_ = nullptr;
+    unsigned* d_gw_ = nullptr;
+    void* d_sort_tmp_ = nullptr;
+    size_t sort_tmp_bytes_ = 0;
+    unsigned* d_combo_ = nullptr;        // [P * K * K]
+    unsigned* d_plane_start_ = nullptr;  // [P]
+    unsigned* d_item_off_ = nullptr;     // [P + 1]
+    int4* d_items_ = nullptr;
+    Summary* d_summary_ = nullptr;
+    Summary* h_summary_ = nullptr;       // pinned
+    hipEvent_t ev_[5] = {};
+    bool timed_ = false;
+};
+
+// Zero-fill with a kernel (capture-safe, unlike hipMemsetAsync).
+void fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
+void fill_zero_u32(unsigned* p, size_t n, hipStream_t s);
+
+}  // namespace gw

@@ -1,0 +1,154 @@
+/*
+ * Extended C ABI of the MI355X W-projection gridder (libfpga_gridding_amd.so).
+ *
+ * oskar_grid_wproj.h holds the drop-in entry points with the reference's
+ * exact signatures.  This header adds what a caller that keeps data on the
+ * GPU needs, in the reference's own conventions (OSKAR-style trailing
+ * `int* status`, negative codes from utils/oskar_global.h:43-64):
+ *
+ *   - *_status variants of the drop-in calls;
+ *   - a last-error query for the void drop-in calls;
+ *   - a resident plan: the W-kernel cube, its derived tables and the
+ *     workspace stay on the device across calls, and points, visibilities
+ *     and the grid are passed as DEVICE pointers (no PCIe in the call);
+ *   - device-memory helpers so that callers need no other GPU runtime;
+ *   - per-stage device timings of the last call (HIP events on the plan's
+ *     stream).
+ *
+ * Nothing here has a counterpart in the reference beyond the entry points
+ * cited per function.
+ */
+#ifndef FPGA_GRIDDING_AMD_H_
+#define FPGA_GRIDDING_AMD_H_
+
+#include <stddef.h>
+#include "oskar_grid_wproj.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: the reference's values (utils/oskar_global.h:46-64). */
+enum gw_status_code
+{
+    GW_OK                         = 0,
+    GW_ERR_INVALID_ARGUMENT       = -3,   /* OSKAR_ERR_INVALID_ARGUMENT */
+    GW_ERR_FUNCTION_NOT_AVAILABLE = -4,   /* OSKAR_ERR_FUNCTION_NOT_AVAILABLE */
+    GW_ERR_OUT_OF_RANGE           = -5,   /* OSKAR_ERR_OUT_OF_RANGE */
+    GW_ERR_MEMORY_ALLOC_FAILURE   = -6,   /* OSKAR_ERR_MEMORY_ALLOC_FAILURE */
+    GW_ERR_MEMORY_COPY_FAILURE    = -7,   /* OSKAR_ERR_MEMORY_COPY_FAILURE */
+    GW_ERR_KERNEL_LAUNCH_FAILURE  = -18,  /* OSKAR_ERR_KERNEL_LAUNCH_FAILURE */
+    GW_ERR_COMPUTE_DEVICES        = -19   /* OSKAR_ERR_COMPUTE_DEVICES */
+};
+
+/* Precision of a plan (values of OSKAR_SINGLE / OSKAR_DOUBLE). */
+enum gw_precision { GW_SINGLE = 0x01, GW_DOUBLE = 0x02 };
+
+/* ---------------------------------------------------------------------- */
+/* Status-returning drop-in calls                                          */
+/* ---------------------------------------------------------------------- */
+
+/* oskar_grid_wproj_f (reference oskar_grid_wproj.h:113-131) with a trailing
+ * OSKAR status: returns at once if *status != 0 on entry; on failure sets
+ * *status and leaves num_skipped, norm and grid untouched. */
+GW_EXPORT
+void oskar_grid_wproj_f_status(const size_t num_w_planes, const int* support,
+        const int oversample, const int conv_size_half,
+        const float* conv_func, const size_t num_points, const float* uu,
+        const float* vv, const float* ww, const float* vis,
+        const float* weight, const float cell_size_rad, const float w_scale,
+        const int grid_size, size_t* num_skipped, double* norm, float* grid,
+        int* status);
+
+/* oskar_grid_wproj_d (reference oskar_grid_wproj.h:68-86) with status. */
+GW_EXPORT
+void oskar_grid_wproj_d_status(const size_t num_w_planes, const int* support,
+        const int oversample, const int conv_size_half,
+        const double* conv_func, const size_t num_points, const double* uu,
+        const double* vv, const double* ww, const double* vis,
+        const double* weight, const double cell_size_rad,
+        const double w_scale, const int grid_size, size_t* num_skipped,
+        double* norm, double* grid, int* status);
+
+/* Code of the last failed void drop-in call on this thread (0 if the last
+ * call succeeded), and a message for any status code. */
+GW_EXPORT int oskar_grid_wproj_last_error(void);
+GW_EXPORT const char* gw_error_string(int status);
+/* Detail text of the last error on this thread (HIP error names etc.). */
+GW_EXPORT const char* gw_last_error_detail(void);
+
+/* Release the per-thread device buffers the drop-in calls keep between
+ * calls (they are otherwise held until the thread exits). */
+GW_EXPORT void gw_release_cached(void);
+
+/* ---------------------------------------------------------------------- */
+/* Resident plan: device-resident W-kernels, device-pointer data           */
+/* ---------------------------------------------------------------------- */
+
+typedef struct gw_plan gw_plan;
+
+/* Create a plan on the current HIP device.  Arguments have the meaning of
+ * the reference gridder's (oskar_grid_wproj.h:50-63); conv_func is a HOST
+ * array of 2*num_w_planes*conv_size_half^2 values of the plan's precision
+ * (float for GW_SINGLE, double for GW_DOUBLE) in the reference layout
+ * conv_func[2*((p*C + iy)*C + ix) + {0,1}].  For GW_SINGLE, cell_size_rad
+ * and w_scale are rounded to float as the FP32 gridder receives them. */
+GW_EXPORT
+gw_plan* gw_plan_create(int precision, size_t num_w_planes,
+        const int* support, int oversample, int conv_size_half,
+        const void* conv_func, double cell_size_rad, double w_scale,
+        int grid_size, int* status);
+
+GW_EXPORT void gw_plan_free(gw_plan* plan);
+
+/* Use the given hipStream_t for the plan's work (NULL: the plan's own
+ * non-blocking stream).  The caller keeps ownership of the stream. */
+GW_EXPORT void gw_plan_set_stream(gw_plan* plan, void* hip_stream);
+
+/* Grid num_points points held in DEVICE memory into a DEVICE grid of
+ * 2*grid_size^2 values of the plan's precision (accumulated, row-major
+ * complex).  d_vis holds 2*num_points values; weights are not used, as in
+ * the reference.  Asynchronous on the plan's stream: read num_skipped and
+ * the norm increment with gw_plan_results(). */
+GW_EXPORT
+void gw_grid_device(gw_plan* plan, size_t num_points, const void* d_uu,
+        const void* d_vv, const void* d_ww, const void* d_vis, void* d_grid,
+        int* status);
+
+/* Wait for the plan's last gw_grid_device call and return its skip count,
+ * the increment to add to the caller's norm, and the number of grid
+ * updates (sum over gridded points of (2*support+1)^2). */
+GW_EXPORT
+void gw_plan_results(gw_plan* plan, size_t* num_skipped, double* norm_delta,
+        unsigned long long* num_updates, int* status);
+
+/* Device milliseconds of the stages of the last gw_grid_device call,
+ * measured with HIP events on the plan's stream:
+ *   ms[0] locate (index math, skip count, histograms, sort keys)
+ *   ms[1] sort   (radix sort of the point keys)
+ *   ms[2] gather (sorted point records) + work planner
+ *   ms[3] grid   (the scatter kernel)
+ *   ms[4] total  (first to last event)
+ * Returns the number of values written (<= max_n). */
+GW_EXPORT int gw_plan_timings(gw_plan* plan, double* ms, int max_n);
+
+/* ---------------------------------------------------------------------- */
+/* Device memory helpers (current HIP device)                              */
+/* ---------------------------------------------------------------------- */
+
+GW_EXPORT int   gw_device_count(void);
+GW_EXPORT void* gw_device_malloc(size_t bytes, int* status);
+GW_EXPORT void  gw_device_free(void* ptr);
+GW_EXPORT void  gw_memcpy_h2d(void* dst, const void* src, size_t bytes,
+        int* status);
+GW_EXPORT void  gw_memcpy_d2h(void* dst, const void* src, size_t bytes,
+        int* status);
+GW_EXPORT void  gw_memset_device(void* ptr, int value, size_t bytes,
+        int* status);
+GW_EXPORT void  gw_device_synchronize(int* status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FPGA_GRIDDING_AMD_H_ */

@@ -1,0 +1,297 @@
+"""Pins of the CPU oracle (oracle/gridref.c).
+
+1. Bit-exact against the golden fixtures written from the gcc build of the
+   reference's own sources (tests/golden/make_golden.py).
+2. Bit-exact against that build directly, where it is present
+   (oracle/_ref/libref_gridder.so), over several shapes: oversample 1, 3,
+   4, 8, support 0, skips at the edges.
+3. Closed forms and invariants that a plausible mistake in the scatter
+   (a transposed index, a dropped |.|, a wrong conjugation sign, a
+   footprint off by one) would break.
+"""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from fpga_gridding_amd import synth
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+
+
+def _bits_equal(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.dtype == np.float32:
+        return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    if a.dtype == np.float64:
+        return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    return np.array_equal(a, b)
+
+
+# --------------------------------------------------------------------------
+# 1. Golden fixtures (reference build outputs)
+# --------------------------------------------------------------------------
+
+def test_golden_f_diagnostics():
+    d = _load("ref_f_diag")
+    wl = synth.workload_from_arrays(d)
+    g, sk, nrm, per_s, per_w, per_cell, avg_s = \
+        oracle.grid_wproj_f_diagnostics(wl, max_support=d["per_support"].size,
+                                        max_plane=d["per_plane"].size)
+    assert sk == int(d["num_skipped"])
+    assert nrm == float(d["norm"])
+    assert _bits_equal(g, d["grid"])
+    assert np.array_equal(per_s, d["per_support"])
+    assert np.array_equal(per_w, d["per_plane"])
+    assert np.array_equal(per_cell, d["per_cell"])
+    assert _bits_equal(avg_s, d["avg_support"])
+    # The plain FP32 oracle has the same arithmetic (oskar_grid_wproj.c:276).
+    g2, sk2, nrm2 = oracle.grid_wproj_f(wl)
+    assert sk2 == sk and nrm2 == nrm and _bits_equal(g2, g)
+
+
+def test_golden_f_10000_edge_cases():
+    d = _load("ref_f_10000")
+    wl = synth.workload_from_arrays(d)
+    assert wl.num_points == 10000
+    g, sk, nrm = oracle.grid_wproj_f(wl)
+    assert sk == int(d["num_skipped"])
+    assert nrm == float(d["norm"])
+    assert _bits_equal(g, d["grid"])
+
+
+def test_golden_d():
+    d = _load("ref_d")
+    wl = synth.workload_from_arrays(d)
+    g, sk, nrm = oracle.grid_wproj_d(wl)
+    assert sk == int(d["num_skipped"])
+    assert nrm == float(d["norm"])
+    assert _bits_equal(g, d["grid"])
+
+
+def test_golden_weights():
+    d = _load("ref_weights")
+    G = int(d["grid_size"])
+    cell = float(d["cell_size_rad"])
+    wg = np.zeros(G * G, np.float32)
+    sk = oracle.grid_weights_write_f(d["uu"], d["vv"], d["weight"], cell, G,
+                                     wg)
+    assert sk == int(d["skipped_write_f"])
+    assert _bits_equal(wg, d["grid_f"])
+    wo = np.zeros(d["uu"].size, np.float32)
+    sk = oracle.grid_weights_read_f(d["uu"], d["vv"], d["weight"], wo, cell,
+                                    G, wg)
+    assert sk == int(d["skipped_read_f"])
+    assert _bits_equal(wo, d["weight_out_f"])
+
+
+# --------------------------------------------------------------------------
+# 2. Against the reference build, where present
+# --------------------------------------------------------------------------
+
+REF_CASES = [
+    dict(n=4000, grid_size=64, num_planes=5, s_min=1, s_max=5, ov=4, seed=1),
+    dict(n=3000, grid_size=48, num_planes=4, s_min=0, s_max=4, ov=1, seed=2),
+    dict(n=3000, grid_size=50, num_planes=3, s_min=2, s_max=6, ov=3, seed=3),
+    dict(n=2000, grid_size=96, num_planes=7, s_min=1, s_max=7, ov=8, seed=4),
+    dict(n=2000, grid_size=33, num_planes=1, s_min=3, s_max=3, ov=4, seed=5),
+]
+
+
+@pytest.mark.skipif(not oracle.ref_available(),
+                    reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("case", REF_CASES, ids=lambda c: f"ov{c['ov']}-s{c['seed']}")
+def test_against_reference_build(case):
+    ref = oracle.ref()
+    wl = synth.random_small(**case)
+    g, sk, nrm, *h = oracle.grid_wproj_f_diagnostics(wl)
+    gr, skr, nrmr, *hr = oracle.grid_wproj_f_diagnostics(wl, lib_=ref)
+    assert sk == skr and nrm == nrmr and _bits_equal(g, gr)
+    for a, b in zip(h, hr):
+        assert _bits_equal(a, b)
+    gd, skd, nrmd = oracle.grid_wproj_d(wl)
+    gdr, skdr, nrmdr = oracle.grid_wproj_d(wl, lib_=ref)
+    assert skd == skdr and nrmd == nrmdr and _bits_equal(gd, gdr)
+
+
+# --------------------------------------------------------------------------
+# 3. Closed forms and invariants
+# --------------------------------------------------------------------------
+
+def _ones_cube(wl):
+    cf = np.zeros_like(wl.conv_func)
+    cf[0::2] = 1.0
+    return cf
+
+
+def test_all_ones_gives_footprint_counts():
+    """vis = 1+0i and every tap 1+0i (the reference's FAKE_* switches,
+    Defines.h:37-38): grid_re counts covering footprints, grid_im = 0,
+    norm = sum (2S+1)^2."""
+    wl = synth.random_small(n=1500, grid_size=40, num_planes=4, s_min=0,
+                            s_max=4, seed=21)
+    wl = dataclasses.replace(wl, conv_func=_ones_cube(wl),
+                             vis=np.tile([1.0, 0.0], wl.num_points)
+                             .astype(np.float32))
+    g, sk, nrm = oracle.grid_wproj_f(wl)
+    rec, U = oracle.locate_f(wl)
+    G = wl.grid_size
+    count = np.zeros((G, G), np.int64)
+    ok = rec["skip"] == 0
+    for gu, gv, s in zip(rec["grid_u"][ok], rec["grid_v"][ok],
+                         rec["support"][ok]):
+        count[gv - s:gv + s + 1, gu - s:gu + s + 1] += 1
+    g = g.reshape(G, G, 2)
+    assert np.array_equal(g[..., 0].astype(np.int64), count)
+    assert not np.any(g[..., 1])
+    assert nrm == float(np.sum((2 * rec["support"][ok] + 1) ** 2))
+    assert nrm == float(U)
+    assert sk == int(np.sum(~ok))
+
+
+def test_nearest_cell_histogram_matches_weights_grid():
+    """S = 0, oversample 1, unit taps: the gridder reduces to the
+    nearest-cell histogram of oskar_grid_weights_write_f
+    (utils/oskar_grid_weights.c:106-138) with unit weights."""
+    wl = synth.random_small(n=2500, grid_size=32, num_planes=1, s_min=0,
+                            s_max=0, ov=1, seed=22, edge_fraction=0.0)
+    wl = dataclasses.replace(wl, conv_func=_ones_cube(wl),
+                             vis=np.tile([1.0, 0.0], wl.num_points)
+                             .astype(np.float32))
+    g, sk, _ = oracle.grid_wproj_f(wl)
+    G = wl.grid_size
+    hist = np.zeros(G * G, np.float32)
+    sk_w = oracle.grid_weights_write_f(wl.uu, wl.vv, np.ones(wl.num_points),
+                                       wl.cell_size_rad, G, hist)
+    assert sk == sk_w
+    assert np.array_equal(g[0::2], hist)
+
+
+def test_linearity_in_vis_fp64():
+    wl = synth.random_small(n=800, grid_size=48, num_planes=4, seed=23)
+    rng = np.random.default_rng(0)
+    v1 = rng.standard_normal(2 * wl.num_points)
+    v2 = rng.standard_normal(2 * wl.num_points)
+    g1, _, _ = oracle.grid_wproj_d(dataclasses.replace(wl, vis=v1))
+    g2, _, _ = oracle.grid_wproj_d(dataclasses.replace(wl, vis=v2))
+    g12, _, _ = oracle.grid_wproj_d(dataclasses.replace(wl, vis=2.5 * v1 - v2))
+    np.testing.assert_allclose(g12, 2.5 * g1 - g2, rtol=0,
+                               atol=1e-12 * np.abs(g12).max())
+
+
+def test_w_sign_conjugates_taps():
+    """Flipping the sign of w (w != 0) conjugates every tap
+    (oskar_grid_wproj.c:172, :233): with real vis, grid_im flips sign and
+    grid_re is unchanged, bit for bit."""
+    wl = synth.random_small(n=1200, grid_size=48, num_planes=4, seed=24)
+    ww = np.where(wl.ww == 0, np.float32(1.0), wl.ww)
+    vis = wl.vis.copy()
+    vis[1::2] = 0.0
+    a = dataclasses.replace(wl, ww=ww, vis=vis)
+    b = dataclasses.replace(wl, ww=-ww, vis=vis)
+    ga, ska, na = oracle.grid_wproj_f(a)
+    gb, skb, nb = oracle.grid_wproj_f(b)
+    assert ska == skb and na == nb
+    assert _bits_equal(ga[0::2], gb[0::2])
+    # Equal values (untouched cells are +0 and -0 after negation).
+    assert np.array_equal(ga[1::2], -gb[1::2])
+    assert np.any(ga[1::2] != 0)
+
+
+def test_w_zero_is_not_conjugated():
+    """w = 0 is not > 0, so conv_conj = +1 (oskar_grid_wproj.c:172): same
+    result as a small negative w on the same plane."""
+    wl = synth.random_small(n=500, grid_size=40, num_planes=3, seed=25)
+    z = dataclasses.replace(wl, ww=np.zeros_like(wl.ww))
+    m = dataclasses.replace(wl, ww=np.full_like(wl.ww, -1e-30))
+    gz, _, _ = oracle.grid_wproj_f(z)
+    gm, _, _ = oracle.grid_wproj_f(m)
+    assert _bits_equal(gz, gm)
+
+
+def test_norm_does_not_depend_on_vis():
+    wl = synth.random_small(n=1000, grid_size=48, num_planes=4, seed=26)
+    _, _, n1 = oracle.grid_wproj_f(wl)
+    _, _, n2 = oracle.grid_wproj_f(dataclasses.replace(wl, vis=wl.vis * 3))
+    assert n1 == n2
+
+
+def test_accumulates_grid_and_norm_resets_skipped():
+    """grid and norm are accumulated, num_skipped is reset
+    (oskar_grid_wproj.c:153, :236-241)."""
+    wl = synth.random_small(n=600, grid_size=40, num_planes=3, seed=27)
+    g, sk, nrm = oracle.grid_wproj_d(wl)
+    g2, sk2, nrm2 = oracle.grid_wproj_d(wl, grid=g.copy(), norm=nrm)
+    assert sk2 == sk
+    np.testing.assert_allclose(g2, 2 * g, rtol=0,
+                               atol=1e-13 * np.abs(g).max())
+    assert abs(nrm2 - 2 * nrm) <= 1e-12 * abs(nrm)
+
+
+def test_edge_skip_rule():
+    """A footprint touching the first or last row/column is kept; one that
+    crosses it is skipped (oskar_grid_wproj.c:207-212)."""
+    wl = synth.edge_cases()
+    rec, _ = oracle.locate_f(wl)
+    G = wl.grid_size
+    s = rec["support"].astype(np.int64)
+    gu = rec["grid_u"].astype(np.int64)
+    gv = rec["grid_v"].astype(np.int64)
+    expect = (gu + s >= G) | (gu - s < 0) | (gv + s >= G) | (gv - s < 0)
+    assert np.array_equal(rec["skip"].astype(bool), expect)
+    finite = np.isfinite(wl.uu) & np.isfinite(wl.vv)
+    assert expect[~finite].all()
+    # Both outcomes occur at each edge.
+    assert expect.any() and (~expect).any()
+
+
+def test_round_half_away_from_zero():
+    """Ties round away from zero (roundf, oskar_grid_wproj.c:175-176)."""
+    wl = synth.edge_cases()
+    rec, _ = oracle.locate_f(wl)
+    gs = np.float32(wl.grid_size) * np.float32(wl.cell_size_rad)
+    pu = -wl.uu.astype(np.float64) * float(gs)
+    tie = np.isfinite(pu) & (np.abs(pu - np.floor(pu) - 0.5) == 0)
+    assert tie.sum() >= 10
+    expect = np.sign(pu[tie]) * np.floor(np.abs(pu[tie]) + 0.5)
+    assert np.array_equal(rec["grid_u"][tie] - wl.grid_size // 2,
+                          expect.astype(np.int64))
+
+
+def test_plane_index_ties_and_clamp():
+    """plane = round(sqrt(|w| * w_scale)), ties away from zero, clamped to
+    num_w_planes - 1 only for the lookups (oskar_grid_wproj.c:173,
+    :199-204)."""
+    wl = synth.edge_cases()
+    rec, _ = oracle.locate_f(wl)
+    w = np.abs(wl.ww.astype(np.float64))
+    for m in range(3):
+        sel = w == (m + 0.5) ** 2
+        assert sel.any()
+        assert np.all(rec["grid_w"][sel] == m + 1)
+    big = np.isfinite(w) & (w > 100.0)
+    assert np.all(rec["plane"][big] == wl.num_w_planes - 1)
+
+
+def test_update_count():
+    wl = synth.random_small(n=900, grid_size=48, num_planes=5, seed=28)
+    rec, U = oracle.locate_f(wl)
+    ok = rec["skip"] == 0
+    assert U == int(np.sum((2 * rec["support"][ok].astype(np.int64) + 1) ** 2))
+
+
+def test_cells_bruteforce_matches_scatter():
+    """gridref_cells_f (per-cell float64 brute force) agrees with the FP64
+    scatter on every cell."""
+    wl = synth.random_small(n=400, grid_size=32, num_planes=3, seed=29)
+    gd, _, _ = oracle.grid_wproj_d(wl)
+    cells = np.arange(wl.grid_size ** 2, dtype=np.int64)
+    out = oracle.cells_f(wl, cells)
+    np.testing.assert_allclose(out, gd, rtol=0, atol=1e-12 * np.abs(gd).max())
