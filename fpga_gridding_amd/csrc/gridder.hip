@@ -8,23 +8,26 @@
 //              (oskar_grid_wproj.c:169-212), skip count, per-(plane,
 //              sub-cell) histogram, footprint bounding box, sort key
 //              (plane, Morton(u0, v0)) and a 16/32-byte point record;
-//   sort       radix sort of the keys (rocPRIM), points grouped by plane and
+//   sort       radix sort of the keys (rocPRIM): points grouped by plane and
 //              ordered along a Z-curve inside each plane;
-//   k_gather   sorted point records (position, position mod W, sub-cell,
-//              visibility pre-multiplied by the conjugation sign);
-//   k_plan     per-plane counts and offsets, norm increment
-//              (sum of histogram x per-(plane, sub-cell) tap sums), number of
-//              grid updates;
-//   k_items    wave work items: (plane, run of sorted points, block of
-//              footprint cells);
-//   k_grid     the scatter.  Each wave64 owns a block of the (2S+1)^2
-//              footprint-cell residues of one plane (Romein's scheme): lane
-//              slot (rx, ry) always feeds the grid cell congruent to
-//              (rx, ry) mod W inside the current footprint, so it keeps its
-//              sum in registers while consecutive points leave that cell in
-//              place and flushes with a float atomic add only when it moves.
-//              Consecutive points are Z-curve neighbours, so flushes are a
-//              small fraction of the taps.
+//   k_plan     points gridded, update count and norm increment (histogram x
+//              per-(plane, sub-cell) tap sums);
+//   k_gather   sorted point records (position, position mod W, sub-cell
+//              table, visibility pre-multiplied by the conjugation sign) and
+//              the heads of the tile buckets: runs of points of one plane
+//              whose footprint origins share an aligned T x T tile (T per
+//              plane, a power of two, so tiles are Z-curve runs);
+//   scans      bucket starts and work items (<= L points of one bucket);
+//   k_grid     the scatter, one work item per workgroup.  Lane slots own
+//              footprint-cell residues (Romein's scheme): slot (rx, ry)
+//              always feeds the grid cell congruent to (rx, ry) mod W inside
+//              the current footprint, so it sums in registers while
+//              consecutive points leave that cell in place.  When the
+//              footprint moves the cell elsewhere the slot adds its sum into
+//              an LDS copy of the bucket's (T + W - 1)^2 region -- a plain
+//              read-modify-write, since each region cell belongs to exactly
+//              one residue and so to one lane.  At the end of the item the
+//              region is added to the grid in HBM with coalesced atomics.
 #include "gridder_internal.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -86,6 +89,14 @@ __device__ __forceinline__ unsigned long long to_size_x86(T x)
     return 0ull;
 }
 
+// Bits needed to hold values 0..n-1 (at least 1).
+__host__ __device__ __forceinline__ int bits_plane_(int n)
+{
+    int b = 1;
+    while ((1 << b) < n) ++b;
+    return b;
+}
+
 __device__ __forceinline__ unsigned long long spread_bits16(unsigned x)
 {
     unsigned long long v = x & 0xFFFFull;
@@ -140,6 +151,8 @@ __global__ void k_init(Summary* __restrict__ s, unsigned* __restrict__ combo,
         s->v_max = INT_MIN;
         s->u_min = INT_MAX;
         s->u_max = INT_MIN;
+        s->nv32 = 0;
+        s->n_buckets = 0;
         s->total_items = 0;
         s->flags = 0;
     }
@@ -169,7 +182,7 @@ __global__ void __launch_bounds__(256) k_locate(
         size_t n, const T* __restrict__ uu, const T* __restrict__ vv,
         const T* __restrict__ ww, const T* __restrict__ vis, T grid_scale,
         T w_scale, int ov, int G, int P, const int* __restrict__ support,
-        int h, int K, int coord_bits,
+        int h, int K, int tile_bits, int combo_bits, int key_bits,
         unsigned long long* __restrict__ keys, unsigned* __restrict__ idx,
         typename Types<T>::Rec* __restrict__ rec,
         unsigned* __restrict__ combo, Summary* __restrict__ summary,
@@ -188,8 +201,12 @@ __global__ void __launch_bounds__(256) k_locate(
 
     const int centre = G / 2;
     const T ovf = (T)ov;
-    const unsigned long long skip_key =
-            (unsigned long long)P << (2 * coord_bits);
+    // Sort key, most significant first: tile (Z-order), plane, sub-cell
+    // table, position inside the tile (Z-order).  Skipped points sort last.
+    const unsigned long long skip_key = 1ull << key_bits;
+    const int sh_combo = 2 * tile_bits;
+    const int sh_plane = sh_combo + combo_bits;
+    const int sh_tile = sh_plane + bits_plane_(P);
     int nskip = 0;
     int vmin = INT_MAX, vmax = INT_MIN, umin = INT_MAX, umax = INT_MIN;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -230,12 +247,16 @@ __global__ void __launch_bounds__(256) k_locate(
             const int off_v = to_int_x86(d_round((rv - pv) * ovf));
             const int u0 = (int)gu - S, v0 = (int)gv - S;
             const int conj = w > (T)0 ? 1 : 0;
-            key = ((unsigned long long)plane << (2 * coord_bits)) |
-                    spread_bits16((unsigned)u0) |
-                    (spread_bits16((unsigned)v0) << 1);
             const int iu = min(max(off_u + h, 0), K - 1);
             const int iv = min(max(off_v + h, 0), K - 1);
-            const int c = (plane * K + iv) * K + iu;
+            const int table = iv * K + iu;
+            const int c = plane * K * K + table;
+            const unsigned long long mort = spread_bits16((unsigned)u0) |
+                    (spread_bits16((unsigned)v0) << 1);
+            key = ((mort >> sh_combo) << sh_tile) |
+                    ((unsigned long long)plane << sh_plane) |
+                    ((unsigned long long)table << sh_combo) |
+                    (mort & ((1ull << sh_combo) - 1ull));
             if (lds_hist) atomicAdd(&s_combo[c], 1u);
             else atomicAdd(&combo[c], 1u);
             vmin = min(vmin, v0);
@@ -295,87 +316,135 @@ __global__ void __launch_bounds__(256) k_locate(
         }
 }
 
-// Sorted point records.  meta = (u0 | v0 << 16, u0 mod W | v0 mod W << 16,
-// sub-cell/conj/plane, grid_w); vq = (re, -s*im, im, s*re) with s = -1 when
-// ww > 0 (oskar_grid_wproj.c:172, :233), so that
-//   grid_re += vq.x * c_re + vq.y * c_im,  grid_im += vq.z * c_re + vq.w * c_im
-// with (c_re, c_im) read from the cube unmodified.
-template <typename T>
-__global__ void __launch_bounds__(256) k_gather(
-        size_t n, const Summary* __restrict__ summary,
-        const unsigned* __restrict__ idx,
-        const typename Types<T>::Rec* __restrict__ rec,
-        const PlaneInfo* __restrict__ pinfo, const unsigned* __restrict__ gw,
-        int4* __restrict__ meta, typename Types<T>::T4* __restrict__ vq)
+// -------------------------------------------------------------------------
+// Device-wide exclusive scan of u32 (three kernels; the element count is
+// read from device memory so the sequence needs no host round trip)
+// -------------------------------------------------------------------------
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Exclusive scan of one value per thread across the block; *total gets the
+// block sum.  s_tmp holds >= 17 words.
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v,
+        unsigned* s_tmp, unsigned* total)
 {
-    const size_t nv = n - summary->skipped;
-    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < nv;
-         s += (size_t)gridDim.x * blockDim.x)
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    unsigned x = v;
+    for (int o = 1; o < 64; o <<= 1)
     {
-        const unsigned i = idx[s];
-        const typename Types<T>::Rec r = rec[i];
-        const int plane = (unsigned)r.sub >> 17;
-        const unsigned W = (unsigned)pinfo[plane].W;
-        const unsigned u0 = (unsigned)r.pos & 0xFFFFu;
-        const unsigned v0 = (unsigned)r.pos >> 16;
-        const T sgn = ((r.sub >> 16) & 1) ? (T)-1 : (T)1;
-        meta[s] = make_int4(r.pos, (int)((u0 % W) | ((v0 % W) << 16)), r.sub,
-                gw ? (int)min(gw[i], 0x7FFFFFFFu) : 0);
-        typename Types<T>::T4 q;
-        q.x = r.re;
-        q.y = -sgn * r.im;
-        q.z = r.im;
-        q.w = sgn * r.re;
-        vq[s] = q;
+        const unsigned y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[wid] = x;
+    __syncthreads();
+    if (wid == 0)
+    {
+        const unsigned w = lane < nwaves ? s_tmp[lane] : 0u;
+        unsigned wi = w;
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const unsigned y = __shfl_up(wi, o, 64);
+            if (lane >= o) wi += y;
+        }
+        if (lane < nwaves) s_tmp[lane] = wi - w;
+        if (lane == nwaves - 1) s_tmp[16] = wi;
+    }
+    __syncthreads();
+    const unsigned r = x - v + s_tmp[wid];
+    *total = s_tmp[16];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_up(
+        const unsigned* __restrict__ in, const unsigned* __restrict__ count,
+        unsigned* __restrict__ partials)
+{
+    __shared__ unsigned s_tmp[17];
+    const unsigned n = *count;
+    const size_t base = (size_t)blockIdx.x * kScanTile +
+            (size_t)threadIdx.x * kScanItems;
+    unsigned sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) sum += in[base + k];
+    unsigned total;
+    (void)block_exclusive_scan(sum, s_tmp, &total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_mid(
+        unsigned* __restrict__ partials, const unsigned* __restrict__ count,
+        unsigned* __restrict__ total_out)
+{
+    __shared__ unsigned s_tmp[17];
+    const unsigned n = *count;
+    const unsigned nb = (n + kScanTile - 1) / kScanTile;
+    unsigned carry = 0;
+    for (unsigned base = 0; base < nb; base += kScanThreads)
+    {
+        const unsigned i = base + threadIdx.x;
+        const unsigned v = i < nb ? partials[i] : 0u;
+        unsigned total;
+        const unsigned ex = block_exclusive_scan(v, s_tmp, &total);
+        if (i < nb) partials[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_down(
+        const unsigned* __restrict__ in, const unsigned* __restrict__ count,
+        const unsigned* __restrict__ partials, unsigned* __restrict__ out)
+{
+    __shared__ unsigned s_tmp[17];
+    const unsigned n = *count;
+    const size_t base = (size_t)blockIdx.x * kScanTile +
+            (size_t)threadIdx.x * kScanItems;
+    unsigned v[kScanItems];
+    unsigned sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        sum += v[k];
+    }
+    unsigned total;
+    unsigned run = block_exclusive_scan(sum, s_tmp, &total) +
+            partials[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
     }
 }
 
-// Per-plane offsets, work-item counts, update count and norm increment.
-// One block of 1024 threads; tiles of 1024 planes with a running carry.
+// -------------------------------------------------------------------------
+// Planner, gather and work items
+// -------------------------------------------------------------------------
+
+// Points gridded, update count and norm increment.  One block of 1024.
 __global__ void __launch_bounds__(1024) k_plan(
         int P, int K2, const unsigned* __restrict__ combo,
-        const PlaneInfo* __restrict__ pinfo, const double* __restrict__ normtab,
-        unsigned* __restrict__ plane_start, unsigned* __restrict__ item_off,
-        unsigned items_cap, Summary* __restrict__ summary)
+        const PlaneInfo* __restrict__ pinfo,
+        const double* __restrict__ normtab, Summary* __restrict__ summary)
 {
-    __shared__ unsigned s_a[1024];
-    __shared__ unsigned s_b[1024];
     __shared__ double s_d[1024];
     __shared__ unsigned long long s_u[1024];
     const int t = threadIdx.x;
-    unsigned carry_pts = 0, carry_items = 0;
-    unsigned long long upd = 0;
-    for (int base = 0; base < P; base += 1024)
+    unsigned long long pts = 0, upd = 0;
+    for (int p = t; p < P; p += 1024)
     {
-        const int p = base + t;
-        unsigned cnt = 0, items = 0;
-        if (p < P)
-        {
-            for (int c = 0; c < K2; ++c) cnt += combo[p * K2 + c];
-            const PlaneInfo pi = pinfo[p];
-            items = ((cnt + pi.L - 1) / pi.L) * pi.nw;
-            upd += (unsigned long long)cnt * pi.W * pi.W;
-        }
-        s_a[t] = cnt;
-        s_b[t] = items;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1)
-        {
-            const unsigned a = t >= o ? s_a[t - o] : 0u;
-            const unsigned b = t >= o ? s_b[t - o] : 0u;
-            __syncthreads();
-            s_a[t] += a;
-            s_b[t] += b;
-            __syncthreads();
-        }
-        if (p < P)
-        {
-            plane_start[p] = carry_pts + s_a[t] - cnt;
-            item_off[p] = carry_items + s_b[t] - items;
-        }
-        carry_pts += s_a[1023];
-        carry_items += s_b[1023];
-        __syncthreads();
+        unsigned cnt = 0;
+        for (int c = 0; c < K2; ++c) cnt += combo[p * K2 + c];
+        const unsigned long long W = (unsigned long long)pinfo[p].W;
+        pts += cnt;
+        upd += (unsigned long long)cnt * W * W;
     }
     // Norm increment: fixed-order reduction (deterministic).
     double part = 0.0;
@@ -393,198 +462,438 @@ __global__ void __launch_bounds__(1024) k_plan(
         }
         __syncthreads();
     }
+    const unsigned long long upd_total = s_u[0];
+    __syncthreads();
+    s_u[t] = pts;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1)
+    {
+        if (t < o) s_u[t] += s_u[t + o];
+        __syncthreads();
+    }
     if (t == 0)
     {
-        item_off[P] = carry_items;
-        summary->n_valid = carry_pts;
-        summary->updates = s_u[0];
+        summary->n_valid = s_u[0];
+        summary->nv32 = (unsigned)s_u[0];
+        summary->updates = upd_total;
         summary->norm_delta = s_d[0];
-        if (carry_items > items_cap)
-        {
-            summary->flags |= 2u;
-            summary->total_items = items_cap;
-        }
-        else
-        {
-            summary->total_items = carry_items;
-        }
     }
 }
 
-// Expand per-plane work into wave items (plane, first point, points,
-// first footprint cell).  One block per plane.
-__global__ void __launch_bounds__(256) k_items(
-        int P, const PlaneInfo* __restrict__ pinfo,
-        const unsigned* __restrict__ plane_start,
-        const unsigned* __restrict__ item_off,
-        const Summary* __restrict__ summary, unsigned items_cap,
-        int4* __restrict__ items)
+// Sorted point records.  meta = (u0 | v0 << 16, sub-cell table
+// (off_v + h) * K + (off_u + h), plane, grid_w); vq = (re, -s*im, im, s*re)
+// with s = -1 when ww > 0 (oskar_grid_wproj.c:172, :233), so that
+//   grid_re += vq.x * c_re + vq.y * c_im,  grid_im += vq.z * c_re + vq.w * c_im
+// with (c_re, c_im) the tap as stored in the kernel cube.  head[s] = 1 where
+// a new tile starts (key bits above tile_shift).
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather(
+        const Summary* __restrict__ summary,
+        const unsigned long long* __restrict__ keys,
+        const unsigned* __restrict__ idx,
+        const typename Types<T>::Rec* __restrict__ rec,
+        const unsigned* __restrict__ gw, int h, int K, int tile_shift,
+        int4* __restrict__ meta, typename Types<T>::T4* __restrict__ vq,
+        unsigned* __restrict__ head)
 {
-    const int p = blockIdx.x;
-    if (p >= P) return;
-    const PlaneInfo pi = pinfo[p];
-    const unsigned start = plane_start[p];
-    const unsigned end = (p + 1 < P) ? plane_start[p + 1] :
-            (unsigned)summary->n_valid;
-    const unsigned cnt = end - start;
-    const unsigned off = item_off[p];
-    const unsigned n_items = item_off[p + 1] - off;
-    for (unsigned j = threadIdx.x; j < n_items; j += blockDim.x)
+    const unsigned nv = summary->nv32;
+    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
+         s += gridDim.x * blockDim.x)
     {
-        const unsigned chunk = j / pi.nw;
-        const unsigned wv = j - chunk * pi.nw;
-        const unsigned first = chunk * pi.L;
-        const unsigned len = min((unsigned)pi.L, cnt - first);
-        if (off + j < items_cap)
-            items[off + j] = make_int4(p, (int)(start + first), (int)len,
-                    (int)(wv * pi.cpw));
+        const unsigned long long key = keys[s];
+        head[s] = (s == 0 || (keys[s - 1] >> tile_shift) !=
+                (key >> tile_shift)) ? 1u : 0u;
+        const unsigned i = idx[s];
+        const typename Types<T>::Rec r = rec[i];
+        const int plane = (unsigned)r.sub >> 17;
+        const T sgn = ((r.sub >> 16) & 1) ? (T)-1 : (T)1;
+        const int off_u = (r.sub & 0xFF) - 128;
+        const int off_v = ((r.sub >> 8) & 0xFF) - 128;
+        const int table = (off_v + h) * K + (off_u + h);
+        meta[s] = make_int4(r.pos, table, plane,
+                gw ? (int)min(gw[i], 0x7FFFFFFFu) : 0);
+        typename Types<T>::T4 q;
+        q.x = r.re;
+        q.y = -sgn * r.im;
+        q.z = r.im;
+        q.w = sgn * r.re;
+        vq[s] = q;
     }
 }
 
-// The scatter (oskar_grid_wproj.c:216-240).  Persistent: each wave walks
-// the work items with a grid-wide stride.  A wave item covers footprint
-// cells [c0, c0 + cpw) of plane p; lane slot s owns cell
-// c = c0 + lane + 64*s, i.e. residue (rx, ry) = (c mod W, c / W).  For a
-// point whose footprint starts at (u0, v0), that residue lands on grid cell
-// (u0 + dx, v0 + dy) with dx = (rx - u0) mod W, dy = (ry - v0) mod W, tap
-// offset k = dx - S, j = dy - S, tap
-//   conv_func[p][|off_v + j*ov|][|off_u + k*ov|].
-// The slot accumulates while its grid cell stays the same and flushes with
-// an atomic add when the footprint moves it elsewhere.
-template <typename T, bool DIAG>
-__global__ void __launch_bounds__(256) k_grid(
-        const Summary* __restrict__ summary, const int4* __restrict__ items,
-        const PlaneInfo* __restrict__ pinfo, const int4* __restrict__ meta,
-        const typename Types<T>::T4* __restrict__ vq,
-        const typename Types<T>::T2* __restrict__ cube, int C, int ov, int G,
-        int row0, T* __restrict__ grid, unsigned* __restrict__ diag_count,
-        unsigned long long* __restrict__ diag_wsum)
+// bstart[b] = first sorted point of tile bucket b; bstart[n_buckets] = nv.
+__global__ void __launch_bounds__(256) k_buckets(
+        const Summary* __restrict__ summary, const unsigned* __restrict__ head,
+        const unsigned* __restrict__ bpos, unsigned* __restrict__ bstart)
+{
+    const unsigned nv = summary->nv32;
+    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
+         s += gridDim.x * blockDim.x)
+        if (head[s]) bstart[bpos[s]] = s;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        bstart[summary->n_buckets] = nv;
+}
+
+// Jobs per tile bucket: ceil(points / job_points).
+__global__ void __launch_bounds__(256) k_bucket_items(
+        const Summary* __restrict__ summary,
+        const unsigned* __restrict__ bstart, unsigned job_points,
+        unsigned* __restrict__ icnt)
+{
+    const unsigned nb = summary->n_buckets;
+    for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b < nb;
+         b += gridDim.x * blockDim.x)
+        icnt[b] = (bstart[b + 1] - bstart[b] + job_points - 1) / job_points;
+}
+
+__device__ __forceinline__ unsigned compact_bits16(unsigned long long x)
+{
+    x &= 0x55555555ull;
+    x = (x | (x >> 1)) & 0x33333333ull;
+    x = (x | (x >> 2)) & 0x0F0F0F0Full;
+    x = (x | (x >> 4)) & 0x00FF00FFull;
+    x = (x | (x >> 8)) & 0x0000FFFFull;
+    return (unsigned)x;
+}
+
+// Job = (tile origin u | v << 16, first sorted point, points, 0).
+__global__ void __launch_bounds__(256) k_write_items(
+        const Summary* __restrict__ summary,
+        const unsigned long long* __restrict__ keys,
+        const unsigned* __restrict__ bstart,
+        const unsigned* __restrict__ ioff, int tile_bits, int tile_shift,
+        unsigned job_points, unsigned items_cap, int4* __restrict__ items)
+{
+    const unsigned nb = summary->n_buckets;
+    for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b < nb;
+         b += gridDim.x * blockDim.x)
+    {
+        const unsigned start = bstart[b], end = bstart[b + 1];
+        const unsigned long long tile = keys[start] >> tile_shift;
+        const unsigned ux0 = compact_bits16(tile) << tile_bits;
+        const unsigned vy0 = compact_bits16(tile >> 1) << tile_bits;
+        const int origin = (int)(ux0 | (vy0 << 16));
+        unsigned o = ioff[b];
+        for (unsigned first = start; first < end; first += job_points, ++o)
+            if (o < items_cap)
+                items[o] = make_int4(origin, (int)first,
+                        (int)min(job_points, end - first), 0);
+    }
+}
+
+// -------------------------------------------------------------------------
+// The scatter (oskar_grid_wproj.c:216-240), one job per workgroup
+// -------------------------------------------------------------------------
+//
+// A job is up to kJobPoints sorted points of one T x T tile of footprint
+// origins, ordered by plane, then sub-cell table, then position.  The
+// workgroup accumulates them into an LDS copy of the tile's
+// (T + Wmax - 1)^2 grid region (planar real / imaginary arrays), across
+// planes, and adds the region to the grid in HBM once at the end of the job.
+//
+// Inside a plane segment the waves form streams of wpp waves; each stream
+// takes a contiguous share of the segment's points.  Lane slot s of wave w
+// (w-th in its stream) owns footprint cell c = w*64 + lane + s*64*wpp,
+// i.e. offset (j, k) = (c / W, c mod W) from the footprint origin, whose tap
+// for the point's sub-cell table is etab[table][c]
+// (= conv_func[p][|off_v + (j-S)*ov|][|off_u + (k-S)*ov|]).  The tap stays in
+// registers while consecutive points share the plane and table; the product
+// with the visibility stays in registers while they share the position;
+// when the position changes the slot adds its sum to the region with an LDS
+// atomic.  For planes whose region would not fit in LDS the sums go straight
+// to the grid in HBM.
+
+template <typename T>
+struct ScatterCtx
+{
+    typedef typename Types<T>::T2 T2;
+    typedef typename Types<T>::T4 T4;
+    const int4* __restrict__ meta;
+    const T4* __restrict__ vq;
+    const T2* __restrict__ etab;
+    const PlaneInfo* __restrict__ pinfo;
+    double* reg_re;                // LDS region, real parts (FP64)
+    double* reg_im;                // LDS region, imaginary parts (FP64)
+    T* __restrict__ grid;
+    unsigned* __restrict__ diag_count;
+    unsigned long long* __restrict__ diag_wsum;
+    int R, G, row0, ux0, vy0;
+    bool lds;
+};
+
+// The LDS region is FP64: on gfx950 ds_add_f64 sustains about 20 times the
+// throughput of ds_add_f32 (measured: ~7 vs ~0.33 lane-ops per clock per
+// CU), and the tile's sums are then carried in double until the one add
+// into the grid per job.
+__device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
+
+// One stream's share [a, b) of a plane segment: footprint cells
+// [cb, cb + 64 * wpp * NS), NS cells per lane.
+template <typename T, bool DIAG, int NS>
+__device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
+        const PlaneInfo& pi, int wis, int a, int b, int cb)
 {
     typedef typename Types<T>::T2 T2;
     typedef typename Types<T>::T4 T4;
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int waves_per_block = blockDim.x >> 6;
-    const unsigned total = summary->total_items;
-    const unsigned stride = gridDim.x * waves_per_block;
-    for (unsigned it = blockIdx.x * waves_per_block + wave; it < total;
-         it += stride)
+    const int W = pi.W, W2 = W * W;
+    const int stride = 64 * pi.wpp;
+    const T2* __restrict__ ptab = x.etab + pi.eoff;
+    const int cstride = x.lds ? x.R : x.G;
+    int cidx[NS];
+    unsigned loff[NS];     // cell offset from the footprint origin
+    bool valid[NS];
+    T tre[NS], tim[NS];
+    T acc_re[NS], acc_im[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
     {
-        const int4 item = items[it];
-        const int p = item.x;
-        const PlaneInfo pi = pinfo[p];
-        const int W = pi.W, S = pi.S, nslot = pi.nslot;
-        const int cend = min(item.w + pi.cpw, W * W);
-        const T2* __restrict__ taps = cube + (size_t)p * C * C;
-
-        int rx[kSlotsMax], ry[kSlotsMax];
-        bool valid[kSlotsMax];
-        T acc_re[kSlotsMax], acc_im[kSlotsMax];
-        unsigned cur[kSlotsMax];
-        unsigned cnt[kSlotsMax];
-        unsigned long long wsum[kSlotsMax];
-        int kxo[kSlotsMax], kyo[kSlotsMax];
+        const int c = cb + wis * 64 + lane + s * stride;
+        valid[s] = c < W2;
+        cidx[s] = valid[s] ? c : 0;
+        const int j = cidx[s] / W;
+        const int k = cidx[s] - j * W;
+        loff[s] = (unsigned)(j * cstride + k);
+        tre[s] = (T)0;
+        tim[s] = (T)0;
+        acc_re[s] = (T)0;
+        acc_im[s] = (T)0;
+    }
+    unsigned cnt = 0;
+    unsigned long long wsum = 0;
+    int cur_table = -1;
+    unsigned cur_pos = 0xFFFFFFFFu;
+    unsigned base = 0;
+    for (int k = a; k < b; ++k)
+    {
+        const int4 m = x.meta[k];
+        const T4 v = x.vq[k];
+        if (m.y != cur_table)
+        {
+            cur_table = m.y;
+            const T2* __restrict__ tp = ptab + (size_t)m.y * (size_t)W2;
 #pragma unroll
-        for (int s = 0; s < kSlotsMax; ++s)
-        {
-            const int c = item.w + lane + 64 * s;
-            valid[s] = (s < nslot) && (c < cend);
-            ry[s] = c / W;
-            rx[s] = c - ry[s] * W;
-            acc_re[s] = (T)0;
-            acc_im[s] = (T)0;
-            cur[s] = kNoCell;
-            cnt[s] = 0;
-            wsum[s] = 0;
-            kxo[s] = 0;
-            kyo[s] = 0;
-        }
-
-        unsigned prev = 0xFFFFFFFFu;
-        const int k_end = item.y + item.z;
-        for (int k = item.y; k < k_end; ++k)
-        {
-            const int4 m = meta[k];
-            const T4 v = vq[k];
-            if ((unsigned)m.x != prev)
+            for (int s = 0; s < NS; ++s)
             {
-                prev = (unsigned)m.x;
-                const int u0 = m.x & 0xFFFF;
-                const int v0 = (int)((unsigned)m.x >> 16);
-                const int u0m = m.y & 0xFFFF;
-                const int v0m = (int)((unsigned)m.y >> 16);
-                const int vrow = v0 - row0;
+                const T2 t = tp[cidx[s]];
+                tre[s] = valid[s] ? t.x : (T)0;
+                tim[s] = valid[s] ? t.y : (T)0;
+            }
+        }
+        if ((unsigned)m.x != cur_pos)
+        {
+            if (cur_pos != 0xFFFFFFFFu)
+            {
 #pragma unroll
-                for (int s = 0; s < kSlotsMax; ++s)
+                for (int s = 0; s < NS; ++s)
                 {
-                    if (s < nslot && valid[s])
+                    if (!valid[s]) continue;
+                    if (x.lds)
                     {
-                        int dx = rx[s] - u0m;
-                        dx += dx < 0 ? W : 0;
-                        int dy = ry[s] - v0m;
-                        dy += dy < 0 ? W : 0;
-                        const unsigned g = (unsigned)(vrow + dy) * (unsigned)G +
-                                (unsigned)(u0 + dx);
-                        if (g != cur[s])
+                        lds_add(x.reg_re + base + loff[s], (double)acc_re[s]);
+                        lds_add(x.reg_im + base + loff[s], (double)acc_im[s]);
+                    }
+                    else
+                    {
+                        const size_t g = (size_t)base + loff[s];
+                        atomic_add_real(x.grid + 2 * g, acc_re[s]);
+                        atomic_add_real(x.grid + 2 * g + 1, acc_im[s]);
+                        if (DIAG)
                         {
-                            if (cur[s] != kNoCell)
-                            {
-                                atomic_add_real(grid + 2 * (size_t)cur[s],
-                                        acc_re[s]);
-                                atomic_add_real(grid + 2 * (size_t)cur[s] + 1,
-                                        acc_im[s]);
-                                if (DIAG)
-                                {
-                                    atomicAdd(&diag_count[cur[s]], cnt[s]);
-                                    atomicAdd(&diag_wsum[cur[s]], wsum[s]);
-                                }
-                            }
-                            cur[s] = g;
-                            acc_re[s] = (T)0;
-                            acc_im[s] = (T)0;
-                            cnt[s] = 0;
-                            wsum[s] = 0;
+                            atomicAdd(&x.diag_count[g], cnt);
+                            atomicAdd(&x.diag_wsum[g], wsum);
                         }
-                        kxo[s] = (dx - S) * ov;
-                        kyo[s] = (dy - S) * ov;
                     }
+                    acc_re[s] = (T)0;
+                    acc_im[s] = (T)0;
                 }
             }
-            const int off_u = (m.z & 0xFF) - 128;
-            const int off_v = ((m.z >> 8) & 0xFF) - 128;
-#pragma unroll
-            for (int s = 0; s < kSlotsMax; ++s)
-            {
-                if (s < nslot && valid[s])
-                {
-                    const int ix = abs(off_u + kxo[s]);
-                    const int iy = abs(off_v + kyo[s]);
-                    const T2 c = taps[iy * C + ix];
-                    acc_re[s] = fma(v.x, c.x, acc_re[s]);
-                    acc_re[s] = fma(v.y, c.y, acc_re[s]);
-                    acc_im[s] = fma(v.z, c.x, acc_im[s]);
-                    acc_im[s] = fma(v.w, c.y, acc_im[s]);
-                    if (DIAG)
-                    {
-                        cnt[s] += 1;
-                        wsum[s] += (unsigned long long)S + (unsigned)m.w;
-                    }
-                }
-            }
+            cur_pos = (unsigned)m.x;
+            cnt = 0;
+            wsum = 0;
+            const int u0 = m.x & 0xFFFF;
+            const int v0 = (int)((unsigned)m.x >> 16);
+            base = x.lds ? (unsigned)((v0 - x.vy0) * x.R + (u0 - x.ux0)) :
+                    (unsigned)((v0 - x.row0) * x.G + u0);
         }
 #pragma unroll
-        for (int s = 0; s < kSlotsMax; ++s)
+        for (int s = 0; s < NS; ++s)
         {
-            if (s < nslot && valid[s] && cur[s] != kNoCell)
+            acc_re[s] = fma(v.x, tre[s], acc_re[s]);
+            acc_re[s] = fma(v.y, tim[s], acc_re[s]);
+            acc_im[s] = fma(v.z, tre[s], acc_im[s]);
+            acc_im[s] = fma(v.w, tim[s], acc_im[s]);
+        }
+        if (DIAG)
+        {
+            cnt += 1;
+            wsum += (unsigned long long)pi.S + (unsigned)m.w;
+        }
+    }
+    if (cur_pos != 0xFFFFFFFFu)
+    {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            if (!valid[s]) continue;
+            if (x.lds)
             {
-                atomic_add_real(grid + 2 * (size_t)cur[s], acc_re[s]);
-                atomic_add_real(grid + 2 * (size_t)cur[s] + 1, acc_im[s]);
+                lds_add(x.reg_re + base + loff[s], (double)acc_re[s]);
+                lds_add(x.reg_im + base + loff[s], (double)acc_im[s]);
+            }
+            else
+            {
+                const size_t g = (size_t)base + loff[s];
+                atomic_add_real(x.grid + 2 * g, acc_re[s]);
+                atomic_add_real(x.grid + 2 * g + 1, acc_im[s]);
                 if (DIAG)
                 {
-                    atomicAdd(&diag_count[cur[s]], cnt[s]);
-                    atomicAdd(&diag_wsum[cur[s]], wsum[s]);
+                    atomicAdd(&x.diag_count[g], cnt);
+                    atomicAdd(&x.diag_wsum[g], wsum);
                 }
             }
+        }
+    }
+}
+
+template <typename T, bool DIAG>
+__global__ void __launch_bounds__(kGridThreads) k_grid(
+        const Summary* __restrict__ summary, const int4* __restrict__ items,
+        const PlaneInfo* __restrict__ pinfo, const int4* __restrict__ meta,
+        const typename Types<T>::T4* __restrict__ vq,
+        const typename Types<T>::T2* __restrict__ etab, int G, int row0,
+        int R, T* __restrict__ grid, int force_direct,
+        unsigned* __restrict__ diag_count,
+        unsigned long long* __restrict__ diag_wsum)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ unsigned s_seg[kJobPoints + 1];   // segment starts
+    __shared__ int s_segp[kJobPoints];           // segment planes
+    __shared__ unsigned s_tmp[17];
+    __shared__ unsigned s_nseg;
+    const int tid = threadIdx.x;
+    const int NT = blockDim.x;
+    const int nwaves = NT >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool lds = !force_direct && R > 0;
+    ScatterCtx<T> x;
+    x.meta = meta;
+    x.vq = vq;
+    x.etab = etab;
+    x.pinfo = pinfo;
+    x.reg_re = reinterpret_cast<double*>(lds_raw);
+    x.reg_im = x.reg_re + (lds ? R * R : 0);
+    x.grid = grid;
+    x.diag_count = diag_count;
+    x.diag_wsum = diag_wsum;
+    x.R = R;
+    x.G = G;
+    x.row0 = row0;
+    x.lds = lds;
+    if (lds)
+    {
+        for (int i = tid; i < 2 * R * R; i += NT) x.reg_re[i] = 0.0;
+    }
+    const unsigned total = summary->total_items;
+    for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
+    {
+        const int4 job = items[it];
+        x.ux0 = job.x & 0xFFFF;
+        x.vy0 = (int)((unsigned)job.x >> 16);
+        const int start = job.y;
+        const int len = job.z;
+        // Plane segments of the job (points are sorted by plane).
+        unsigned heads = 0;
+        int myplane[2] = {0, 0};
+        bool myhead[2] = {false, false};
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+        {
+            const int i = 2 * tid + q;
+            if (i < len)
+            {
+                myplane[q] = meta[start + i].z;
+                myhead[q] = (i == 0) || meta[start + i - 1].z != myplane[q];
+                heads += myhead[q] ? 1u : 0u;
+            }
+        }
+        unsigned nseg;
+        unsigned pos = block_exclusive_scan(heads, s_tmp, &nseg);
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+        {
+            if (myhead[q])
+            {
+                s_seg[pos] = (unsigned)(2 * tid + q);
+                s_segp[pos] = myplane[q];
+                ++pos;
+            }
+        }
+        if (tid == 0)
+        {
+            s_seg[nseg] = (unsigned)len;
+            s_nseg = nseg;
+        }
+        __syncthreads();
+        // Values read back from LDS are the same in every lane; make that
+        // explicit so that the point loop runs on scalar registers and
+        // scalar loads.
+        nseg = (unsigned)__builtin_amdgcn_readfirstlane((int)s_nseg);
+        for (unsigned g = 0; g < nseg; ++g)
+        {
+            const int plane = __builtin_amdgcn_readfirstlane(s_segp[g]);
+            const PlaneInfo pi = pinfo[plane];
+            const int a = start +
+                    __builtin_amdgcn_readfirstlane((int)s_seg[g]);
+            const int b = start +
+                    __builtin_amdgcn_readfirstlane((int)s_seg[g + 1]);
+            const int streams = nwaves / pi.wpp;
+            const int stream = wave / pi.wpp;
+            const int wis = wave - stream * pi.wpp;
+            if (stream >= streams) continue;
+            const int share = (b - a + streams - 1) / streams;
+            const int sa = a + stream * share;
+            const int sb = min(b, sa + share);
+            if (sa >= sb) continue;
+            const int pass = 64 * pi.wpp * pi.nslot;
+            for (int cb = 0; cb < pi.W * pi.W; cb += pass)
+            {
+                switch (pi.nslot)
+                {
+                case 1: scatter_stream<T, DIAG, 1>(x, pi, wis, sa, sb, cb); break;
+                case 2: scatter_stream<T, DIAG, 2>(x, pi, wis, sa, sb, cb); break;
+                case 3: scatter_stream<T, DIAG, 3>(x, pi, wis, sa, sb, cb); break;
+                case 4: scatter_stream<T, DIAG, 4>(x, pi, wis, sa, sb, cb); break;
+                case 5: scatter_stream<T, DIAG, 5>(x, pi, wis, sa, sb, cb); break;
+                case 6: scatter_stream<T, DIAG, 6>(x, pi, wis, sa, sb, cb); break;
+                case 7: scatter_stream<T, DIAG, 7>(x, pi, wis, sa, sb, cb); break;
+                default: scatter_stream<T, DIAG, 8>(x, pi, wis, sa, sb, cb); break;
+                }
+            }
+        }
+        __syncthreads();
+        if (lds)
+        {
+            // Region -> grid, coalesced along rows, clearing it for the next
+            // job.  Cells no footprint reached hold zero and are skipped
+            // (they may lie off the grid).
+            for (int i = tid; i < R * R; i += NT)
+            {
+                const double re = x.reg_re[i];
+                const double im = x.reg_im[i];
+                if (re != 0.0 || im != 0.0)
+                {
+                    x.reg_re[i] = 0.0;
+                    x.reg_im[i] = 0.0;
+                    const int y = i / R;
+                    const int xx = i - y * R;
+                    const size_t g = (size_t)(x.vy0 - row0 + y) * (size_t)G +
+                            (size_t)(x.ux0 + xx);
+                    atomic_add_real(grid + 2 * g, (T)re);
+                    atomic_add_real(grid + 2 * g + 1, (T)im);
+                }
+            }
+            __syncthreads();
         }
     }
 }
@@ -592,7 +901,6 @@ __global__ void __launch_bounds__(256) k_grid(
 // -------------------------------------------------------------------------
 // Plan
 // -------------------------------------------------------------------------
-
 namespace {
 
 int bits_for(unsigned long long v)
@@ -602,32 +910,58 @@ int bits_for(unsigned long long v)
     return b;
 }
 
+// Expanded tap table: for every plane p and sub-cell offset (off_u, off_v)
+// the (2S+1)^2 taps of one footprint in row-major (j, k) order,
+//   etab[eoff_p + ((off_v+h)*K + off_u+h)*W^2 + (j+S)*W + (k+S)]
+//     = conv_func[p*C^2 + |off_v + j*ov|*C + |off_u + k*ov|]
+// (the reference's flat cube index, oskar_grid_wproj.c:222-233).  Also the
+// per-(plane, sub-cell) tap sums in the reference's order (rows j outer,
+// columns k inner, double accumulation of float taps, :216-238), which are
+// exactly the per-point `sum` the reference adds to the norm.
 template <typename T>
-void compute_normtab(const T* conv, size_t P, const int* support, int ov,
-                     int C, int h, std::vector<double>& tab)
+void build_tables(const T* conv, size_t P, const int* support, int ov, int C,
+                  int h, std::vector<PlaneInfo>& pinfo, std::vector<T>& etab,
+                  std::vector<double>& normtab)
 {
     const int K = 2 * h + 1;
-    tab.assign(P * K * K, 0.0);
+    size_t total = 0;
+    for (size_t p = 0; p < P; ++p)
+    {
+        const size_t W = 2 * (size_t)support[p] + 1;
+        pinfo[p].eoff = total;
+        total += (size_t)K * K * W * W;
+    }
+    if (total * 2 * sizeof(T) > (size_t)8 << 30)
+        fail(kErrInvalidArgument,
+             "support and oversample need more than 8 GiB of tap tables");
+    etab.resize(2 * total);
+    normtab.assign(P * K * K, 0.0);
     const size_t kdim = (size_t)C * C;
     for (size_t p = 0; p < P; ++p)
     {
         const int S = support[p];
+        const size_t W = 2 * (size_t)S + 1;
         for (int iv = 0; iv < K; ++iv)
         {
             for (int iu = 0; iu < K; ++iu)
             {
-                // Same order as the reference's per-point `sum`
-                // (oskar_grid_wproj.c:216-238): rows j outer, columns k.
                 const int off_v = iv - h, off_u = iu - h;
+                T* dst = etab.data() + 2 * (pinfo[p].eoff +
+                        (size_t)(iv * K + iu) * W * W);
                 double sum = 0.0;
                 for (int j = -S; j <= S; ++j)
                 {
                     const size_t t1 = p * kdim +
                             (size_t)std::abs(off_v + j * ov) * C;
                     for (int k = -S; k <= S; ++k)
-                        sum += conv[2 * (t1 + std::abs(off_u + k * ov))];
+                    {
+                        const size_t t = 2 * (t1 + std::abs(off_u + k * ov));
+                        *dst++ = conv[t];
+                        *dst++ = conv[t + 1];
+                        sum += conv[t];
+                    }
                 }
-                tab[(p * K + iv) * K + iu] = sum;
+                normtab[(p * K + iv) * K + iu] = sum;
             }
         }
     }
@@ -699,34 +1033,57 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     gscale_f_ = (float)grid_size * cell_f_;
     gscale_d_ = (double)grid_size * cell_d_;
     coord_bits_ = std::max(1, bits_for((unsigned long long)(G_ - 1)));
-    key_bits_ = 2 * coord_bits_ + bits_for((unsigned long long)P_);
+    combo_bits_ = std::max(1, bits_for((unsigned long long)(K_ * K_ - 1)));
+    plane_bits_ = bits_plane_((int)P_);
 
-    // Work decomposition per plane.
+    // Tile side T = 2^t (<= 64): the largest whose LDS region
+    // (T + Wmax - 1)^2, real and imaginary, fits the budget.  If none does
+    // the scatter adds straight into the grid in HBM.
+    const size_t real = sizeof(double);   // the region is FP64
+    const long long wmax = 2ll * s_max_ + 1;
+    region_ = 0;
+    tile_bits_ = std::min(6, coord_bits_);
+    for (int t = std::min(6, coord_bits_); t >= 0; --t)
+    {
+        const long long R = (1ll << t) + wmax - 1;
+        if ((size_t)(2 * R * R) * real <= (size_t)kRegionBytes)
+        {
+            tile_bits_ = t;
+            region_ = (int)R;
+            break;
+        }
+    }
+    lds_bytes_ = (size_t)2 * region_ * region_ * real;
+    tile_shift_ = 2 * tile_bits_ + combo_bits_ + plane_bits_;
+    key_bits_ = 2 * coord_bits_ + combo_bits_ + plane_bits_;
+
+    // Streams per plane: wpp waves cover the footprint with <= kSlotsMax
+    // cells per lane (larger footprints take several passes).
     pinfo_.resize(P_);
-    items_per_point_ = 0.0;
-    items_fixed_ = 1;
     for (size_t p = 0; p < P_; ++p)
     {
         PlaneInfo& pi = pinfo_[p];
         pi.S = support_[p];
         pi.W = 2 * pi.S + 1;
         const long long cells = (long long)pi.W * pi.W;
-        pi.nw = (int)((cells + 64 * kSlotsMax - 1) / (64 * kSlotsMax));
-        pi.cpw = (int)((cells + pi.nw - 1) / pi.nw);
-        pi.nslot = (pi.cpw + 63) / 64;
-        pi.L = std::max(1, std::min(65535, kTapsPerItem / pi.cpw));
+        pi.wpp = 1;
+        while (pi.wpp < kGridThreads / 64 &&
+                (long long)pi.wpp * 64 * kSlotsMax < cells)
+            pi.wpp *= 2;
+        pi.nslot = (int)std::min<long long>(kSlotsMax,
+                (cells + 64ll * pi.wpp - 1) / (64ll * pi.wpp));
         pi.pad0 = pi.pad1 = 0;
-        items_per_point_ = std::max(items_per_point_, (double)pi.nw / pi.L);
-        items_fixed_ += pi.nw;
+        pi.eoff = 0;
     }
 
     check_hip(hipGetDevice(&device_), kErrComputeDevices, "hipGetDevice");
-    check_hip(hipDeviceGetAttribute(&n_cu_, hipDeviceAttributeMultiprocessorCount,
-            device_), kErrComputeDevices, "hipDeviceGetAttribute");
+    check_hip(hipDeviceGetAttribute(&n_cu_,
+            hipDeviceAttributeMultiprocessorCount, device_),
+            kErrComputeDevices, "hipDeviceGetAttribute");
     int occ = 0;
     check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ,
             precision == 0x01 ? (const void*)k_grid<float, false> :
-            (const void*)k_grid<double, false>, 256, 0),
+            (const void*)k_grid<double, false>, kGridThreads, lds_bytes_),
             kErrComputeDevices, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     grid_blocks_ = n_cu_ * std::max(1, occ);
     check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
@@ -737,27 +1094,37 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
 
     // Constant tables.
     std::vector<double> normtab;
-    const size_t real_size = prec_ == 0x01 ? sizeof(float) : sizeof(double);
-    const size_t cube_bytes = 2 * cube_len * real_size;
+    std::vector<float> etab_f;
+    std::vector<double> etab_d;
+    const void* etab_host = nullptr;
+    size_t etab_bytes = 0;
     if (prec_ == 0x01)
-        compute_normtab(static_cast<const float*>(conv_func), P_,
-                support_.data(), ov_, C_, h_, normtab);
+    {
+        build_tables(static_cast<const float*>(conv_func), P_,
+                support_.data(), ov_, C_, h_, pinfo_, etab_f, normtab);
+        etab_host = etab_f.data();
+        etab_bytes = etab_f.size() * sizeof(float);
+        etab_taps_ = etab_f.size() / 2;
+    }
     else
-        compute_normtab(static_cast<const double*>(conv_func), P_,
-                support_.data(), ov_, C_, h_, normtab);
-    d_cube_ = dmalloc<char>(cube_bytes);
+    {
+        build_tables(static_cast<const double*>(conv_func), P_,
+                support_.data(), ov_, C_, h_, pinfo_, etab_d, normtab);
+        etab_host = etab_d.data();
+        etab_bytes = etab_d.size() * sizeof(double);
+        etab_taps_ = etab_d.size() / 2;
+    }
+    d_etab_ = dmalloc<char>(etab_bytes);
     d_support_ = dmalloc<int>(P_);
     d_pinfo_ = dmalloc<PlaneInfo>(P_);
     d_normtab_ = dmalloc<double>(normtab.size());
     d_combo_ = dmalloc<unsigned>(P_ * K_ * K_);
-    d_plane_start_ = dmalloc<unsigned>(P_);
-    d_item_off_ = dmalloc<unsigned>(P_ + 1);
     d_summary_ = dmalloc<Summary>(1);
     check_hip(hipHostMalloc((void**)&h_summary_, sizeof(Summary),
             hipHostMallocDefault), kErrMemAlloc, "hipHostMalloc");
     std::memset(h_summary_, 0, sizeof(Summary));
-    check_hip(hipMemcpy(d_cube_, conv_func, cube_bytes, hipMemcpyHostToDevice),
-            kErrMemCopy, "upload kernel cube");
+    check_hip(hipMemcpy(d_etab_, etab_host, etab_bytes,
+            hipMemcpyHostToDevice), kErrMemCopy, "upload tap tables");
     check_hip(hipMemcpy(d_support_, support_.data(), P_ * sizeof(int),
             hipMemcpyHostToDevice), kErrMemCopy, "upload support");
     check_hip(hipMemcpy(d_pinfo_, pinfo_.data(), P_ * sizeof(PlaneInfo),
@@ -771,13 +1138,11 @@ Plan::~Plan()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
     free_workspace();
-    dfree(d_cube_);
+    dfree(d_etab_);
     dfree(d_support_);
     dfree(d_pinfo_);
     dfree(d_normtab_);
     dfree(d_combo_);
-    dfree(d_plane_start_);
-    dfree(d_item_off_);
     dfree(d_summary_);
     if (h_summary_) (void)hipHostFree(h_summary_);
     for (auto& e : ev_)
@@ -814,15 +1179,21 @@ void Plan::free_workspace()
         d_keys_[i] = nullptr;
         d_idx_[i] = nullptr;
     }
-    dfree(d_rec_);
+    void** bufs[] = {&d_rec_, &d_vq_, &d_sort_tmp_};
+    for (void** b : bufs)
+    {
+        dfree(*b);
+        *b = nullptr;
+    }
     dfree(d_meta_);
-    dfree(d_vq_);
     dfree(d_gw_);
-    dfree(d_sort_tmp_);
+    dfree(d_head_);
+    dfree(d_bpos_);
+    dfree(d_bstart_);
+    dfree(d_scan_part_);
     dfree(d_items_);
-    d_rec_ = d_vq_ = d_sort_tmp_ = nullptr;
     d_meta_ = nullptr;
-    d_gw_ = nullptr;
+    d_gw_ = d_head_ = d_bpos_ = d_bstart_ = d_scan_part_ = nullptr;
     d_items_ = nullptr;
     cap_ = items_cap_ = 0;
     sort_tmp_bytes_ = 0;
@@ -831,8 +1202,8 @@ void Plan::free_workspace()
 void Plan::ensure_capacity(size_t n)
 {
     if (n <= cap_) return;
-    if (n > 0xFFFFFFFFull)
-        fail(kErrInvalidArgument, "num_points must be below 2^32 per call");
+    if (n >= 0x7FFFFFFFull)
+        fail(kErrInvalidArgument, "num_points must be below 2^31 per call");
     free_workspace();
     const size_t cap = std::max<size_t>(n, 1024);
     const size_t real_size = prec_ == 0x01 ? sizeof(float) : sizeof(double);
@@ -841,22 +1212,39 @@ void Plan::ensure_capacity(size_t n)
         d_keys_[i] = dmalloc<unsigned long long>(cap);
         d_idx_[i] = dmalloc<unsigned>(cap);
     }
-    d_rec_ = dmalloc<char>(cap * (prec_ == 0x01 ? sizeof(RecF) : sizeof(RecD)));
+    d_rec_ = dmalloc<char>(cap *
+            (prec_ == 0x01 ? sizeof(RecF) : sizeof(RecD)));
     d_meta_ = dmalloc<int4>(cap);
     d_vq_ = dmalloc<char>(cap * 4 * real_size);
     d_gw_ = dmalloc<unsigned>(cap);
+    d_head_ = dmalloc<unsigned>(cap);
+    d_bpos_ = dmalloc<unsigned>(cap);
+    d_bstart_ = dmalloc<unsigned>(cap + 1);
+    d_scan_part_ = dmalloc<unsigned>(cap / kScanTile + 2);
     size_t bytes = 0;
-    check_hip(rocprim::radix_sort_pairs(nullptr, bytes, d_keys_[0], d_keys_[1],
-            d_idx_[0], d_idx_[1], cap, 0u, (unsigned)key_bits_, stream_),
-            kErrComputeDevices, "radix sort size query");
+    check_hip(rocprim::radix_sort_pairs(nullptr, bytes, d_keys_[0],
+            d_keys_[1], d_idx_[0], d_idx_[1], cap, 0u, (unsigned)key_bits_ + 1u,
+            stream_), kErrComputeDevices, "radix sort size query");
     d_sort_tmp_ = dmalloc<char>(bytes);
     sort_tmp_bytes_ = bytes;
-    const double bound = (double)cap * items_per_point_ + (double)items_fixed_;
-    if (bound > 2147483647.0)
-        fail(kErrInvalidArgument, "too many work items for one call");
-    items_cap_ = (size_t)bound + 1;
+    // Jobs: one per tile plus one per kJobPoints points.
+    items_cap_ = cap + cap / kJobPoints + 2;
     d_items_ = dmalloc<int4>(items_cap_);
     cap_ = cap;
+}
+
+void Plan::scan_u32(const unsigned* in, unsigned* out, const unsigned* count,
+                    size_t max_n, unsigned* total)
+{
+    const unsigned nb = (unsigned)std::max<size_t>(1,
+            (max_n + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_scan_up, dim3(nb), dim3(kScanThreads), 0, stream_,
+            in, count, d_scan_part_);
+    hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kScanThreads), 0, stream_,
+            d_scan_part_, count, total);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanThreads), 0, stream_,
+            in, count, (const unsigned*)d_scan_part_, out);
+    check_hip(hipGetLastError(), kErrKernelLaunch, "scan launch");
 }
 
 template <typename T>
@@ -870,24 +1258,26 @@ void Plan::locate_t(size_t n, const T* uu, const T* vv, const T* ww,
     hipLaunchKernelGGL(k_init, dim3(std::max(1, std::min(1024,
             (n_combo + 255) / 256))), dim3(256), 0, stream_, d_summary_,
             d_combo_, n_combo);
-    const int blocks = (int)std::min<size_t>((n + 255) / 256,
-            (size_t)n_cu_ * 4);
+    const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
+            (n + 255) / 256, (size_t)n_cu_ * 4));
     const T gscale = sizeof(T) == 4 ? (T)gscale_f_ : (T)gscale_d_;
     const T wsc = sizeof(T) == 4 ? (T)ws_f_ : (T)ws_d_;
     typedef typename Types<T>::Rec Rec;
     if (diag)
-        hipLaunchKernelGGL((k_locate<T, true>), dim3(std::max(blocks, 1)),
-                dim3(256), 0, stream_, n, uu, vv, ww, vis, gscale, wsc, ov_,
-                G_, (int)P_, d_support_, h_, K_, coord_bits_, d_keys_[0],
-                d_idx_[0], (Rec*)d_rec_, d_combo_, d_summary_, d_gw_,
-                diag->per_support, diag->per_plane, diag->plane_cap);
+        hipLaunchKernelGGL((k_locate<T, true>), dim3(blocks), dim3(256), 0,
+                stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
+                d_support_, h_, K_, tile_bits_, combo_bits_, key_bits_,
+                d_keys_[0], d_idx_[0],
+                (Rec*)d_rec_, d_combo_, d_summary_, d_gw_, diag->per_support,
+                diag->per_plane, diag->plane_cap);
     else
-        hipLaunchKernelGGL((k_locate<T, false>), dim3(std::max(blocks, 1)),
-                dim3(256), 0, stream_, n, uu, vv, ww, vis, gscale, wsc, ov_,
-                G_, (int)P_, d_support_, h_, K_, coord_bits_, d_keys_[0],
-                d_idx_[0], (Rec*)d_rec_, d_combo_, d_summary_,
-                (unsigned*)nullptr, (unsigned long long*)nullptr,
-                (unsigned long long*)nullptr, 0);
+        hipLaunchKernelGGL((k_locate<T, false>), dim3(blocks), dim3(256), 0,
+                stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
+                d_support_, h_, K_, tile_bits_, combo_bits_, key_bits_,
+                d_keys_[0], d_idx_[0],
+                (Rec*)d_rec_, d_combo_, d_summary_, (unsigned*)nullptr,
+                (unsigned long long*)nullptr, (unsigned long long*)nullptr,
+                0);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_locate launch");
     check_hip(hipEventRecord(ev_[1], stream_), kErrKernelLaunch, "event");
 }
@@ -898,42 +1288,58 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     typedef typename Types<T>::Rec Rec;
     typedef typename Types<T>::T2 T2;
     typedef typename Types<T>::T4 T4;
-    size_t bytes = sort_tmp_bytes_;
-    check_hip(rocprim::radix_sort_pairs(d_sort_tmp_, bytes, d_keys_[0],
-            d_keys_[1], d_idx_[0], d_idx_[1], n, 0u, (unsigned)key_bits_,
-            stream_), kErrKernelLaunch, "radix sort");
+    if (n > 0)
+    {
+        size_t bytes = sort_tmp_bytes_;
+        check_hip(rocprim::radix_sort_pairs(d_sort_tmp_, bytes, d_keys_[0],
+                d_keys_[1], d_idx_[0], d_idx_[1], n, 0u,
+                (unsigned)key_bits_ + 1u, stream_), kErrKernelLaunch,
+                "radix sort");
+    }
     check_hip(hipEventRecord(ev_[2], stream_), kErrKernelLaunch, "event");
-    const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
-            (n + 255) / 256, (size_t)n_cu_ * 4));
-    hipLaunchKernelGGL((k_gather<T>), dim3(blocks), dim3(256), 0, stream_, n,
-            (const Summary*)d_summary_, (const unsigned*)d_idx_[1],
-            (const Rec*)d_rec_, (const PlaneInfo*)d_pinfo_,
-            (const unsigned*)(diag ? d_gw_ : nullptr), d_meta_, (T4*)d_vq_);
-    check_hip(hipGetLastError(), kErrKernelLaunch, "k_gather launch");
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream_, (int)P_,
             K_ * K_, (const unsigned*)d_combo_, (const PlaneInfo*)d_pinfo_,
-            (const double*)d_normtab_, d_plane_start_, d_item_off_,
-            (unsigned)items_cap_, d_summary_);
-    check_hip(hipGetLastError(), kErrKernelLaunch, "k_plan launch");
-    hipLaunchKernelGGL(k_items, dim3((unsigned)P_), dim3(256), 0, stream_,
-            (int)P_, (const PlaneInfo*)d_pinfo_,
-            (const unsigned*)d_plane_start_, (const unsigned*)d_item_off_,
-            (const Summary*)d_summary_, (unsigned)items_cap_, d_items_);
-    check_hip(hipGetLastError(), kErrKernelLaunch, "k_items launch");
+            (const double*)d_normtab_, d_summary_);
+    const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
+            (n + 255) / 256, (size_t)n_cu_ * 4));
+    hipLaunchKernelGGL((k_gather<T>), dim3(blocks), dim3(256), 0, stream_,
+            (const Summary*)d_summary_,
+            (const unsigned long long*)d_keys_[1], (const unsigned*)d_idx_[1],
+            (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr), h_,
+            K_, tile_shift_, d_meta_, (T4*)d_vq_, d_head_);
+    check_hip(hipGetLastError(), kErrKernelLaunch, "k_gather launch");
+    // Tiles: exclusive scan of the heads, then their starts.
+    scan_u32(d_head_, d_bpos_, &d_summary_->nv32, n, &d_summary_->n_buckets);
+    hipLaunchKernelGGL(k_buckets, dim3(blocks), dim3(256), 0, stream_,
+            (const Summary*)d_summary_, (const unsigned*)d_head_,
+            (const unsigned*)d_bpos_, d_bstart_);
+    // Jobs per tile (reusing head / bpos), scanned to offsets.
+    hipLaunchKernelGGL(k_bucket_items, dim3(blocks), dim3(256), 0, stream_,
+            (const Summary*)d_summary_, (const unsigned*)d_bstart_,
+            (unsigned)kJobPoints, d_head_);
+    scan_u32(d_head_, d_bpos_, &d_summary_->n_buckets, n,
+            &d_summary_->total_items);
+    hipLaunchKernelGGL(k_write_items, dim3(blocks), dim3(256), 0, stream_,
+            (const Summary*)d_summary_,
+            (const unsigned long long*)d_keys_[1], (const unsigned*)d_bstart_,
+            (const unsigned*)d_bpos_, tile_bits_, tile_shift_,
+            (unsigned)kJobPoints, (unsigned)items_cap_, d_items_);
+    check_hip(hipGetLastError(), kErrKernelLaunch, "job launch");
     check_hip(hipEventRecord(ev_[3], stream_), kErrKernelLaunch, "event");
     if (diag)
-        hipLaunchKernelGGL((k_grid<T, true>), dim3(grid_blocks_), dim3(256),
-                0, stream_, (const Summary*)d_summary_,
-                (const int4*)d_items_, (const PlaneInfo*)d_pinfo_,
-                (const int4*)d_meta_, (const T4*)d_vq_, (const T2*)d_cube_,
-                C_, ov_, G_, row0, grid, diag->per_cell, diag->wsum_cell);
+        hipLaunchKernelGGL((k_grid<T, true>), dim3(grid_blocks_),
+                dim3(kGridThreads), 0, stream_,
+                (const Summary*)d_summary_, (const int4*)d_items_,
+                (const PlaneInfo*)d_pinfo_, (const int4*)d_meta_,
+                (const T4*)d_vq_, (const T2*)d_etab_, G_, row0, 0, grid, 1,
+                diag->per_cell, diag->wsum_cell);
     else
-        hipLaunchKernelGGL((k_grid<T, false>), dim3(grid_blocks_), dim3(256),
-                0, stream_, (const Summary*)d_summary_,
-                (const int4*)d_items_, (const PlaneInfo*)d_pinfo_,
-                (const int4*)d_meta_, (const T4*)d_vq_, (const T2*)d_cube_,
-                C_, ov_, G_, row0, grid, (unsigned*)nullptr,
-                (unsigned long long*)nullptr);
+        hipLaunchKernelGGL((k_grid<T, false>), dim3(grid_blocks_),
+                dim3(kGridThreads), lds_bytes_, stream_,
+                (const Summary*)d_summary_, (const int4*)d_items_,
+                (const PlaneInfo*)d_pinfo_, (const int4*)d_meta_,
+                (const T4*)d_vq_, (const T2*)d_etab_, G_, row0, region_, grid,
+                0, (unsigned*)nullptr, (unsigned long long*)nullptr);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_grid launch");
     check_hip(hipEventRecord(ev_[4], stream_), kErrKernelLaunch, "event");
 }

@@ -12,13 +12,16 @@
 
 namespace gw {
 
-// Largest number of footprint cells one lane owns (registers per slot:
-// two accumulators, the owned grid cell, two tap offsets).
+// Threads of a scatter workgroup (16 waves), most footprint cells one lane
+// owns at a time (registers per slot: tap and running sum, complex), and
+// the most points in one scatter job (the per-job plane segment table
+// lives in LDS).
+constexpr int kGridThreads = 1024;
 constexpr int kSlotsMax = 8;
-// Target grid updates per wave work item (load-balance granularity).
-constexpr int kTapsPerItem = 1 << 16;
-// Sentinel for "lane slot owns no grid cell yet".
-constexpr unsigned kNoCell = 0xFFFFFFFFu;
+constexpr int kJobPoints = 2 * kGridThreads;
+// LDS bytes for the job's grid region (planar FP64 real and imaginary
+// arrays; 88 x 88 cells fit).
+constexpr int kRegionBytes = 124 * 1024;
 
 // Per-plane constants, fixed by the support table (host-computed at plan
 // creation, uploaded once).
@@ -26,11 +29,12 @@ struct PlaneInfo
 {
     int W;       // footprint side 2S+1
     int S;       // support
-    int nslot;   // cell slots per lane in a wave work item (<= kSlotsMax)
-    int cpw;     // footprint cells per wave work item
-    int L;       // points per chunk
-    int nw;      // wave work items per chunk (ceil(W^2 / cpw))
+    int wpp;     // waves per stream (1, 2, 4, 8 or 16)
+    int nslot;   // footprint cells per lane per pass (<= kSlotsMax)
     int pad0, pad1;
+    // First tap of this plane in the expanded tap table: K*K sub-cell
+    // tables of W*W taps each, tap (j, k) at [(iv*K + iu)*W*W + j*W + k].
+    unsigned long long eoff;
 };
 
 // Device-side per-call summary (written by the kernels, read back once).
@@ -42,7 +46,9 @@ struct Summary
     double norm_delta;              // sum over gridded points of tap sums
     int v_min, v_max;               // grid rows touched (inclusive)
     int u_min, u_max;               // grid columns touched (inclusive)
-    unsigned total_items;           // wave work items of the scatter
+    unsigned nv32;                  // n_valid as a 32-bit count
+    unsigned n_buckets;             // tile buckets
+    unsigned total_items;           // work items of the scatter
     unsigned flags;                 // bit 0: diagnostics plane index beyond
                                     // its table; bit 1: work items overflow
 };
@@ -128,6 +134,8 @@ private:
             const T* ww, const T* vis, const DiagBuffers* diag);
     template <typename T> void scatter_t(size_t n, T* grid, int row0,
             const DiagBuffers* diag);
+    void scan_u32(const unsigned* in, unsigned* out, const unsigned* count,
+                  size_t max_n, unsigned* total);
 
     int prec_;
     size_t P_;
@@ -135,12 +143,16 @@ private:
     int s_max_;
     float cell_f_, ws_f_, gscale_f_;
     double cell_d_, ws_d_, gscale_d_;
-    int coord_bits_;
-    int key_bits_;
+    int coord_bits_;     // bits of a grid coordinate
+    int combo_bits_;     // bits of a sub-cell table index
+    int plane_bits_;     // bits of a plane index
+    int tile_bits_;      // log2 of the tile side T
+    int tile_shift_;     // key bits below the tile index
+    int key_bits_;       // valid keys are < 2^key_bits_
+    int region_;         // LDS region side T + Wmax - 1 (0: none)
     std::vector<int> support_;
     std::vector<PlaneInfo> pinfo_;
-    double items_per_point_;   // bound on wave items per gridded point
-    size_t items_fixed_;       // plus this many
+    size_t lds_bytes_ = 0;     // dynamic LDS of the scatter kernel
     int device_ = 0;
     int n_cu_ = 0;
     int grid_blocks_ = 0;
@@ -148,7 +160,8 @@ private:
     bool own_stream_ = false;
 
     // Constant device tables.
-    void* d_cube_ = nullptr;
+    void* d_etab_ = nullptr;             // expanded taps (complex, T)
+    size_t etab_taps_ = 0;
     int* d_support_ = nullptr;
     PlaneInfo* d_pinfo_ = nullptr;
     double* d_normtab_ = nullptr;
@@ -165,8 +178,10 @@ private:
     void* d_sort_tmp_ = nullptr;
     size_t sort_tmp_bytes_ = 0;
     unsigned* d_combo_ = nullptr;        // [P * K * K]
-    unsigned* d_plane_start_ = nullptr;  // [P]
-    unsigned* d_item_off_ = nullptr;     // [P + 1]
+    unsigned* d_head_ = nullptr;         // [N] bucket heads / item counts
+    unsigned* d_bpos_ = nullptr;         // [N] bucket index / item offsets
+    unsigned* d_bstart_ = nullptr;       // [N + 1] bucket starts
+    unsigned* d_scan_part_ = nullptr;    // scan partial sums
     int4* d_items_ = nullptr;
     Summary* d_summary_ = nullptr;
     Summary* h_summary_ = nullptr;       // pinned

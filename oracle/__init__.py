@@ -102,6 +102,10 @@ def lib():
             _sz, _i32p, ct.c_int, ct.c_int, _f32p, _sz, _f32p, _f32p, _f32p,
             _f32p, ct.c_float, ct.c_float, ct.c_int, _sz, _i64p, _f64p]
         _lib.gridref_cells_f.restype = None
+        _lib.gridref_oskar_grid_wproj_f_exact.argtypes = [
+            _sz, _i32p, ct.c_int, ct.c_int, _f32p, _sz, _f32p, _f32p, _f32p,
+            _f32p, ct.c_float, ct.c_float, ct.c_int, _szp, _dp, _f64p]
+        _lib.gridref_oskar_grid_wproj_f_exact.restype = None
     return _lib
 
 
@@ -174,6 +178,22 @@ def grid_wproj_d(wl, grid=None, norm=0.0, lib_=None, uvw64=None):
                           float(np.float32(wl.cell_size_rad)),
                           float(np.float32(wl.w_scale)), g, grid, norm)
     return grid, sk, nrm
+
+
+def grid_wproj_f_exact(wl):
+    """FP32 index math with float64 products and sums: the exact value of
+    the FP32 gridder's result.  Returns (grid float64, skipped, norm)."""
+    g = wl.grid_size
+    grid = np.zeros(2 * g * g, dtype=np.float64)
+    skipped = ct.c_size_t(0)
+    nrm = ct.c_double(0.0)
+    lib().gridref_oskar_grid_wproj_f_exact(
+        wl.num_w_planes, _c(wl.support, np.int32), wl.oversample,
+        wl.conv_size_half, _c(wl.conv_func, np.float32), wl.num_points,
+        _c(wl.uu, np.float32), _c(wl.vv, np.float32), _c(wl.ww, np.float32),
+        _c(wl.vis, np.float32), wl.cell_size_rad, wl.w_scale, g,
+        ct.byref(skipped), ct.byref(nrm), grid)
+    return grid, int(skipped.value), float(nrm.value)
 
 
 def grid_wproj_f_diagnostics(wl, max_support=None, max_plane=None,

@@ -211,6 +211,54 @@ EXPORT void gridref_oskar_grid_wproj_d(const size_t num_w_planes,
     }
 }
 
+/* The exact value of what oskar_grid_wproj_f computes: the FP32 index math
+ * of the reference (same plane, cell, sub-cell and taps), but products and
+ * accumulation carried in float64 into a float64 grid.  The FP32 gridders
+ * (the reference's sequential one and any reordered one) differ from this
+ * only by float rounding of their sums, so it measures both sides' error. */
+EXPORT void gridref_oskar_grid_wproj_f_exact(const size_t num_w_planes,
+        const int* support, const int oversample, const int conv_size_half,
+        const float* conv_func, const size_t num_points, const float* uu,
+        const float* vv, const float* ww, const float* vis,
+        const float cell_size_rad, const float w_scale, const int grid_size,
+        size_t* num_skipped, double* norm, double* grid)
+{
+    const float grid_scale = grid_size * cell_size_rad;
+    const size_t kdim = (size_t)conv_size_half * conv_size_half;
+    size_t i;
+    *num_skipped = 0;
+    for (i = 0; i < num_points; ++i)
+    {
+        locate_t r;
+        int j, k;
+        double sum = 0.0;
+        locate_f(uu[i], vv[i], ww[i], grid_scale, w_scale, oversample,
+                grid_size, num_w_planes, support, &r);
+        if (r.skip) { *num_skipped += 1; continue; }
+        for (j = -r.support; j <= r.support; ++j)
+        {
+            const size_t row = ((size_t)(r.grid_v + j)) * grid_size +
+                    r.grid_u;
+            const size_t trow = r.plane * kdim +
+                    (size_t)abs(r.off_v + j * oversample) * conv_size_half;
+            for (k = -r.support; k <= r.support; ++k)
+            {
+                const size_t t = (trow + abs(r.off_u + k * oversample)) << 1;
+                const double c_re = conv_func[t];
+                const double c_im = (double)conv_func[t + 1] *
+                        (r.conj ? -1.0 : 1.0);
+                const size_t g = (row + k) << 1;
+                grid[g]     += (double)vis[2 * i] * c_re -
+                        (double)vis[2 * i + 1] * c_im;
+                grid[g + 1] += (double)vis[2 * i + 1] * c_re +
+                        (double)vis[2 * i] * c_im;
+                sum += c_re;
+            }
+        }
+        *norm += sum;
+    }
+}
+
 /* oskar_grid_wproj.c:245-370.  Histograms are counted before the skip test
  * (:317-318); num_vis_per_wplane uses the UNCLAMPED plane index; both
  * w_support and grid_w are added into avg_wsupport_per_grid (:359-360) and
