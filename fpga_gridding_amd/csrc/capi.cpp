@@ -117,10 +117,7 @@ void run_dropin(int prec, size_t P, const int* support, int ov, int C,
     if (dh && (!dh->per_support || !dh->per_plane || !dh->per_cell ||
             !dh->wsum_cell))
         gw::fail(gw::kErrInvalidArgument, "diagnostics arrays must not be NULL");
-    if (!support || !conv)
-        gw::fail(gw::kErrInvalidArgument, "support and conv_func must not be NULL");
-    if (P < 1 || C < 1)
-        gw::fail(gw::kErrInvalidArgument, "num_w_planes and conv_size_half must be >= 1");
+    gw::validate_args(P, support, ov, C, conv, G);
 
     int dev = 0;
     gw::check_hip(hipGetDevice(&dev), gw::kErrComputeDevices, "hipGetDevice");

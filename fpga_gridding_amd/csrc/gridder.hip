@@ -983,12 +983,9 @@ void dfree(void* p)
 
 }  // namespace
 
-Plan::Plan(int precision, size_t num_w_planes, const int* support,
-           int oversample, int conv_size_half, const void* conv_func,
-           double cell_size_rad, double w_scale, int grid_size)
+void validate_args(size_t num_w_planes, const int* support, int oversample,
+                   int conv_size_half, const void* conv_func, int grid_size)
 {
-    if (precision != 0x01 && precision != 0x02)
-        fail(kErrInvalidArgument, "precision must be GW_SINGLE or GW_DOUBLE");
     if (num_w_planes < 1 || num_w_planes > 32768)
         fail(kErrInvalidArgument, "num_w_planes must be in [1, 32768]");
     if (!support || !conv_func)
@@ -999,6 +996,33 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
         fail(kErrInvalidArgument, "conv_size_half must be >= 1");
     if (grid_size < 1 || grid_size > 65535)
         fail(kErrInvalidArgument, "grid_size must be in [1, 65535]");
+    // Largest flat tap index the gridder can read, per plane:
+    // p*C^2 + (h + S*ov)*C + (h + S*ov).  The reference reads the flat cube
+    // the same way; it must stay inside the cube.
+    const unsigned long long C = (unsigned long long)conv_size_half;
+    const unsigned long long h = (unsigned long long)(oversample + 1) / 2;
+    const unsigned long long cube_len = num_w_planes * C * C;
+    for (size_t p = 0; p < num_w_planes; ++p)
+    {
+        const int S = support[p];
+        if (S < 0 || S > 16383)
+            fail(kErrInvalidArgument, "support values must be in [0, 16383]");
+        const unsigned long long m = h + (unsigned long long)S * oversample;
+        if ((unsigned long long)p * C * C + m * C + m >= cube_len)
+            fail(kErrInvalidArgument,
+                 "conv_size_half too small for the support and oversample "
+                 "(taps would be read beyond the kernel cube)");
+    }
+}
+
+Plan::Plan(int precision, size_t num_w_planes, const int* support,
+           int oversample, int conv_size_half, const void* conv_func,
+           double cell_size_rad, double w_scale, int grid_size)
+{
+    if (precision != 0x01 && precision != 0x02)
+        fail(kErrInvalidArgument, "precision must be GW_SINGLE or GW_DOUBLE");
+    validate_args(num_w_planes, support, oversample, conv_size_half,
+            conv_func, grid_size);
     prec_ = precision;
     P_ = num_w_planes;
     ov_ = oversample;
@@ -1007,24 +1031,7 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     h_ = (ov_ + 1) / 2;
     K_ = 2 * h_ + 1;
     support_.assign(support, support + P_);
-    s_max_ = 0;
-    // Largest flat tap index the gridder can read, per plane:
-    // p*C^2 + (h + S*ov)*C + (h + S*ov).  The reference reads the flat cube
-    // the same way; it must stay inside the cube.
-    const unsigned long long cube_len = (unsigned long long)P_ * C_ * C_;
-    for (size_t p = 0; p < P_; ++p)
-    {
-        const int S = support_[p];
-        if (S < 0 || S > 16383)
-            fail(kErrInvalidArgument, "support values must be in [0, 16383]");
-        s_max_ = std::max(s_max_, S);
-        const unsigned long long m = (unsigned long long)h_ +
-                (unsigned long long)S * ov_;
-        if ((unsigned long long)p * C_ * C_ + m * C_ + m >= cube_len)
-            fail(kErrInvalidArgument,
-                 "conv_size_half too small for the support and oversample "
-                 "(taps would be read beyond the kernel cube)");
-    }
+    s_max_ = *std::max_element(support_.begin(), support_.end());
     cell_d_ = cell_size_rad;
     ws_d_ = w_scale;
     cell_f_ = (float)cell_size_rad;

@@ -91,6 +91,12 @@ struct Error
 [[noreturn]] void fail(int code, const std::string& detail);
 void check_hip(hipError_t e, int code, const char* what);
 
+// Host-side argument checks of a gridding call (no device access): raises
+// kErrInvalidArgument for sizes the gridder cannot handle, including a
+// kernel cube too small for support and oversample.
+void validate_args(size_t num_w_planes, const int* support, int oversample,
+                   int conv_size_half, const void* conv_func, int grid_size);
+
 // Diagnostics outputs (device buffers) of one call.
 struct DiagBuffers
 {
