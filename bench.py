@@ -1,0 +1,218 @@
+"""Benchmark of the MI355X W-projection gridder on the 512-station workload.
+
+One step = one pass of the hot path (oskar_grid_wproj_f semantics) over the
+whole 13,081,600-point batch: index math, sort, work planning and the scatter
+into an 8192^2 grid, with points, visibilities and the grid resident in HBM
+(the plan API: gw_grid_device + gw_plan_results).  Prints ONE JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+The scatter kernel's duration comes from HIP events the library records on
+its own stream around each stage; the roofline uses the algorithmic FLOPs of
+the reference's inner loop (8 per grid update, oskar_grid_wproj.c:236-237)
+over that duration.  The CPU baseline times the reference's own
+oskar_grid_wproj_f (a gcc build of its sources, oracle/_ref) on one host
+core over a random sample of the same points; it is a reported baseline,
+not a target.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as ct
+import dataclasses
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 peak (AMD published)
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = ("512 stations (core + log-radial to 40 km), 130816 baselines x "
+            "100 x 10 s, 150 MHz, 13,081,600 points, grid 8192^2, 5 deg FOV, "
+            "128 w-planes, support 4..36, oversample 4")
+
+
+def _traffic_from_profile():
+    """HBM bytes per scatter launch, from the committed rocprofv3 counter
+    summary (FETCH_SIZE / WRITE_SIZE scaled by a copy-kernel calibration),
+    or None if no such summary exists."""
+    path = os.path.join(ROOT, "profiles", "k_grid_ska512_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch"):
+            try:
+                return float(line.split("=")[1].split()[0])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def cpu_baseline(wl, budget_s=12.0, seed=3):
+    """Time the reference's oskar_grid_wproj_f on one core over random
+    10000-point slices of the workload (its loop handles exactly 10000
+    points per call, oskar_grid_wproj.c:155)."""
+    import oracle
+    lib = oracle.ref()
+    kind = "reference"
+    if lib is None:
+        lib = oracle.lib()
+        kind = "port"
+    fn = lib.oskar_grid_wproj_f if kind == "reference" else \
+        lib.gridref_oskar_grid_wproj_f
+    rng = np.random.default_rng(seed)
+    n = wl.num_points
+    G = wl.grid_size
+    grid = np.zeros(2 * G * G, np.float32)
+    slice_n = 10000
+    support = np.ascontiguousarray(wl.support, np.int32)
+    cube = np.ascontiguousarray(wl.conv_func, np.float32)
+    libc = ct.CDLL(None)
+    done_pts = 0
+    done_upd = 0
+    elapsed = 0.0
+    slices = 0
+    rec_all = None
+    while (elapsed < budget_s or slices < 3) and slices < 4000:
+        idx = np.sort(rng.choice(n, slice_n, replace=False))
+        uu = np.ascontiguousarray(wl.uu[idx])
+        vv = np.ascontiguousarray(wl.vv[idx])
+        ww = np.ascontiguousarray(wl.ww[idx])
+        vis = np.ascontiguousarray(wl.vis.reshape(-1, 2)[idx].reshape(-1))
+        sk = ct.c_size_t(0)
+        nrm = ct.c_double(0.0)
+        # The reference prints a progress line every 1000 points.
+        sys.stdout.flush()
+        saved = os.dup(1)
+        null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(null, 1)
+        try:
+            t0 = time.perf_counter()
+            fn(wl.num_w_planes, support, wl.oversample, wl.conv_size_half,
+               cube, slice_n, uu, vv, ww, vis, None, wl.cell_size_rad,
+               wl.w_scale, G, ct.byref(sk), ct.byref(nrm), grid)
+            elapsed += time.perf_counter() - t0
+        finally:
+            libc.fflush(None)
+            os.dup2(saved, 1)
+            os.close(saved)
+            os.close(null)
+        slices += 1
+        done_pts += slice_n
+        sub = dataclasses.replace(wl, uu=uu, vv=vv, ww=ww, vis=vis)
+        done_upd += oracle.locate_f(sub)[1]
+    return dict(value=done_pts / elapsed / 1e6, unit="Mvis/s", cores=1,
+                kind=kind,
+                sample=f"{slices} random slices of 10000 points "
+                       f"({done_pts} points, {done_upd} grid updates) of the "
+                       f"same workload, oskar_grid_wproj_f single-threaded",
+                updates_per_s=done_upd / elapsed, seconds=elapsed)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not "
+              "supported (the gridder runs on a single MI355X; see the "
+              "sharding notes in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import fpga_gridding_amd as fg
+    from fpga_gridding_amd import synth
+
+    wl = synth.config(1)
+    plan = fg.Plan.for_workload(wl)
+    d_uvw = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww)]
+    d_vis = fg.DeviceArray.from_host(wl.vis)
+    d_grid = fg.DeviceArray(8 * wl.grid_size ** 2)
+    d_grid.zero()
+    fg.device_synchronize()
+
+    def step():
+        plan.grid(wl.num_points, *d_uvw, d_vis, d_grid)
+        return plan.results()
+
+    for _ in range(args.warmup):
+        step()
+    fg.device_synchronize()
+    stage_ms = []
+    step_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        skipped, norm_delta, updates = step()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+        stage_ms.append(plan.timings())
+    fg.device_synchronize()
+    wall = time.perf_counter() - t0
+
+    n = wl.num_points
+    ms_per_step = wall / args.steps * 1e3
+    stage = np.array(stage_ms)          # [steps, 5]
+    grid_ms = float(stage[:, 3].mean())
+    flops = 8.0 * updates
+    achieved = flops / (grid_ms * 1e-3) / 1e12
+    traffic = _traffic_from_profile()
+
+    out = {
+        "metric": "Mvis/s and grid updates/s vs FP32 roofline; p50/p99 "
+                  "job latency at fixed load",
+        "value": n / (ms_per_step * 1e-3) / 1e6,
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "num_points": n,
+                   "grid_size": wl.grid_size,
+                   "num_w_planes": wl.num_w_planes,
+                   "support": [int(wl.support.min()), int(wl.support.max())],
+                   "oversample": wl.oversample,
+                   "inputs": "resident in HBM (plan API)"},
+        "grid_updates_per_s": updates / (ms_per_step * 1e-3),
+        "grid_updates": int(updates),
+        "num_skipped": int(skipped),
+        "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                            "p99": float(np.percentile(step_ms, 99))},
+        "stage_ms_mean": {"locate": float(stage[:, 0].mean()),
+                          "sort": float(stage[:, 1].mean()),
+                          "gather_plan": float(stage[:, 2].mean()),
+                          "scatter": grid_ms,
+                          "total": float(stage[:, 4].mean())},
+        "roofline": {"bound": "mfma", "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic,
+                     "kernel": "k_grid<float> (scatter)",
+                     "compute": "FP32 VALU FMA, no MFMA; peak is MI355X "
+                                "FP32 157.3 TFLOP/s",
+                     "flops_per_launch": flops},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(wl)
+        out["cpu_baseline"] = {k: cb[k] for k in
+                               ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"]["grid_updates_per_s"] = cb["updates_per_s"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
