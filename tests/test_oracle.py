@@ -295,3 +295,17 @@ def test_cells_bruteforce_matches_scatter():
     cells = np.arange(wl.grid_size ** 2, dtype=np.int64)
     out = oracle.cells_f(wl, cells)
     np.testing.assert_allclose(out, gd, rtol=0, atol=1e-12 * np.abs(gd).max())
+
+
+def test_exact_fp32_index_oracle():
+    """gridref_oskar_grid_wproj_f_exact (FP32 index math, float64 sums)
+    agrees with the independent per-cell brute force on every cell, and
+    with the sequential FP32 loop to float rounding."""
+    wl = synth.random_small(n=600, grid_size=32, num_planes=4, seed=30)
+    ge, ske, ne = oracle.grid_wproj_f_exact(wl)
+    gf, skf, nf = oracle.grid_wproj_f(wl)
+    assert ske == skf and abs(ne - nf) <= 1e-12 * abs(nf)
+    cells = np.arange(wl.grid_size ** 2, dtype=np.int64)
+    np.testing.assert_allclose(oracle.cells_f(wl, cells), ge, rtol=0,
+                               atol=1e-12 * np.abs(ge).max())
+    np.testing.assert_allclose(gf, ge, rtol=0, atol=1e-6 * np.abs(ge).max())
