@@ -628,8 +628,13 @@ struct ScatterCtx
 __device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
 
 // One stream's share [a, b) of a plane segment: footprint cells
-// [cb, cb + 64 * wpp * NS), NS cells per lane.
-template <typename T, bool DIAG, int NS>
+// [cb, cb + 64 * wpp * NS), NS cells per lane.  LDS selects the LDS region
+// (else the grid in HBM).  Point records are read two points ahead and the
+// next sub-cell table's taps are loaded before the current point's
+// arithmetic, so both latencies overlap work.  In the LDS path lanes past
+// the footprint are not masked off: their sums go to a per-lane sink cell
+// after the region (kSinkCells of them), which is never read.
+template <typename T, bool DIAG, bool LDS, int NS>
 __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
         const PlaneInfo& pi, int wis, int a, int b, int cb)
 {
@@ -638,84 +643,96 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
     const int lane = threadIdx.x & 63;
     const int W = pi.W, W2 = W * W;
     const int stride = 64 * pi.wpp;
-    const T2* __restrict__ ptab = x.etab + pi.eoff;
-    const int cstride = x.lds ? x.R : x.G;
-    int cidx[NS];
+    const char* __restrict__ ptab =
+            reinterpret_cast<const char*>(x.etab + pi.eoff);
+    const size_t table_bytes = (size_t)W2 * sizeof(T2);
+    const int cstride = LDS ? x.R : x.G;
+    unsigned toff[NS];     // byte offset of the lane's tap in a table
     unsigned loff[NS];     // cell offset from the footprint origin
     bool valid[NS];
-    T tre[NS], tim[NS];
+    T tre[NS], tim[NS], nre[NS], nim[NS];
     T acc_re[NS], acc_im[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
     {
         const int c = cb + wis * 64 + lane + s * stride;
         valid[s] = c < W2;
-        cidx[s] = valid[s] ? c : 0;
-        const int j = cidx[s] / W;
-        const int k = cidx[s] - j * W;
+        const int cc = valid[s] ? c : 0;
+        const int j = cc / W;
+        const int k = cc - j * W;
         loff[s] = (unsigned)(j * cstride + k);
-        tre[s] = (T)0;
-        tim[s] = (T)0;
+        toff[s] = (unsigned)cc * (unsigned)sizeof(T2);
         acc_re[s] = (T)0;
         acc_im[s] = (T)0;
     }
+    auto load_taps = [&](int table, T* re, T* im) {
+        const char* tp = ptab + (size_t)table * table_bytes;
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            const T2 t = *reinterpret_cast<const T2*>(tp + toff[s]);
+            re[s] = t.x;
+            im[s] = t.y;
+        }
+    };
+    auto origin = [&](int pos) -> unsigned {
+        const int u0 = pos & 0xFFFF;
+        const int v0 = (int)((unsigned)pos >> 16);
+        return LDS ? (unsigned)((v0 - x.vy0) * x.R + (u0 - x.ux0)) :
+                (unsigned)((v0 - x.row0) * x.G + u0);
+    };
     unsigned cnt = 0;
     unsigned long long wsum = 0;
-    int cur_table = -1;
-    unsigned cur_pos = 0xFFFFFFFFu;
-    unsigned base = 0;
-    for (int k = a; k < b; ++k)
-    {
-        const int4 m = x.meta[k];
-        const T4 v = x.vq[k];
-        if (m.y != cur_table)
-        {
-            cur_table = m.y;
-            const T2* __restrict__ tp = ptab + (size_t)m.y * (size_t)W2;
+    const unsigned sink = (unsigned)(x.R * x.R + lane);
+    auto flush = [&](unsigned base) {
 #pragma unroll
-            for (int s = 0; s < NS; ++s)
+        for (int s = 0; s < NS; ++s)
+        {
+            if (LDS)
             {
-                const T2 t = tp[cidx[s]];
-                tre[s] = valid[s] ? t.x : (T)0;
-                tim[s] = valid[s] ? t.y : (T)0;
+                const unsigned c = valid[s] ? base + loff[s] : sink;
+                lds_add(x.reg_re + c, (double)acc_re[s]);
+                lds_add(x.reg_im + c, (double)acc_im[s]);
             }
-        }
-        if ((unsigned)m.x != cur_pos)
-        {
-            if (cur_pos != 0xFFFFFFFFu)
+            else if (valid[s])
             {
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
+                const size_t g = (size_t)base + loff[s];
+                atomic_add_real(x.grid + 2 * g, acc_re[s]);
+                atomic_add_real(x.grid + 2 * g + 1, acc_im[s]);
+                if (DIAG)
                 {
-                    if (!valid[s]) continue;
-                    if (x.lds)
-                    {
-                        lds_add(x.reg_re + base + loff[s], (double)acc_re[s]);
-                        lds_add(x.reg_im + base + loff[s], (double)acc_im[s]);
-                    }
-                    else
-                    {
-                        const size_t g = (size_t)base + loff[s];
-                        atomic_add_real(x.grid + 2 * g, acc_re[s]);
-                        atomic_add_real(x.grid + 2 * g + 1, acc_im[s]);
-                        if (DIAG)
-                        {
-                            atomicAdd(&x.diag_count[g], cnt);
-                            atomicAdd(&x.diag_wsum[g], wsum);
-                        }
-                    }
-                    acc_re[s] = (T)0;
-                    acc_im[s] = (T)0;
+                    atomicAdd(&x.diag_count[g], cnt);
+                    atomicAdd(&x.diag_wsum[g], wsum);
                 }
             }
-            cur_pos = (unsigned)m.x;
-            cnt = 0;
-            wsum = 0;
-            const int u0 = m.x & 0xFFFF;
-            const int v0 = (int)((unsigned)m.x >> 16);
-            base = x.lds ? (unsigned)((v0 - x.vy0) * x.R + (u0 - x.ux0)) :
-                    (unsigned)((v0 - x.row0) * x.G + u0);
+            acc_re[s] = (T)0;
+            acc_im[s] = (T)0;
         }
+    };
+
+    int4 m = x.meta[a];
+    T4 v = x.vq[a];
+    int4 mn = m;
+    T4 vn = v;
+    if (a + 1 < b)
+    {
+        mn = x.meta[a + 1];
+        vn = x.vq[a + 1];
+    }
+    load_taps(m.y, tre, tim);
+    unsigned base = origin(m.x);
+    for (int k = a; k < b; ++k)
+    {
+        const bool more = k + 1 < b;
+        int4 mnn = mn;
+        T4 vnn = vn;
+        if (k + 2 < b)
+        {
+            mnn = x.meta[k + 2];
+            vnn = x.vq[k + 2];
+        }
+        const bool new_table = more && mn.y != m.y;
+        if (new_table) load_taps(mn.y, nre, nim);
 #pragma unroll
         for (int s = 0; s < NS; ++s)
         {
@@ -729,30 +746,52 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
             cnt += 1;
             wsum += (unsigned long long)pi.S + (unsigned)m.w;
         }
-    }
-    if (cur_pos != 0xFFFFFFFFu)
-    {
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
+        if (!more) break;
+        if (mn.x != m.x)
         {
-            if (!valid[s]) continue;
-            if (x.lds)
+            flush(base);
+            base = origin(mn.x);
+            cnt = 0;
+            wsum = 0;
+        }
+        if (new_table)
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
             {
-                lds_add(x.reg_re + base + loff[s], (double)acc_re[s]);
-                lds_add(x.reg_im + base + loff[s], (double)acc_im[s]);
-            }
-            else
-            {
-                const size_t g = (size_t)base + loff[s];
-                atomic_add_real(x.grid + 2 * g, acc_re[s]);
-                atomic_add_real(x.grid + 2 * g + 1, acc_im[s]);
-                if (DIAG)
-                {
-                    atomicAdd(&x.diag_count[g], cnt);
-                    atomicAdd(&x.diag_wsum[g], wsum);
-                }
+                tre[s] = nre[s];
+                tim[s] = nim[s];
             }
         }
+        m = mn;
+        v = vn;
+        mn = mnn;
+        vn = vnn;
+    }
+    flush(base);
+}
+
+template <typename T, bool DIAG, bool LDS>
+__device__ __forceinline__ void scatter_dispatch(const ScatterCtx<T>& x,
+        const PlaneInfo& pi, int wis, int a, int b, int cb)
+{
+    // FP64 plans never set nslot above kSlotsMaxF64 (the plan caps it), so
+    // the wider cases are only compiled for FP32.
+    constexpr bool wide = sizeof(T) == 4;
+    switch (pi.nslot)
+    {
+    case 1: scatter_stream<T, DIAG, LDS, 1>(x, pi, wis, a, b, cb); break;
+    case 2: scatter_stream<T, DIAG, LDS, 2>(x, pi, wis, a, b, cb); break;
+    case 3: scatter_stream<T, DIAG, LDS, 3>(x, pi, wis, a, b, cb); break;
+    case 4: scatter_stream<T, DIAG, LDS, 4>(x, pi, wis, a, b, cb); break;
+    case 5: scatter_stream<T, DIAG, LDS, wide ? 5 : 4>(x, pi, wis, a, b, cb);
+        break;
+    case 6: scatter_stream<T, DIAG, LDS, wide ? 6 : 4>(x, pi, wis, a, b, cb);
+        break;
+    case 7: scatter_stream<T, DIAG, LDS, wide ? 7 : 4>(x, pi, wis, a, b, cb);
+        break;
+    default: scatter_stream<T, DIAG, LDS, wide ? 8 : 4>(x, pi, wis, a, b, cb);
+        break;
     }
 }
 
@@ -782,7 +821,7 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
     x.etab = etab;
     x.pinfo = pinfo;
     x.reg_re = reinterpret_cast<double*>(lds_raw);
-    x.reg_im = x.reg_re + (lds ? R * R : 0);
+    x.reg_im = x.reg_re + (lds ? R * R + kSinkCells : 0);
     x.grid = grid;
     x.diag_count = diag_count;
     x.diag_wsum = diag_wsum;
@@ -792,7 +831,8 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
     x.lds = lds;
     if (lds)
     {
-        for (int i = tid; i < 2 * R * R; i += NT) x.reg_re[i] = 0.0;
+        for (int i = tid; i < 2 * (R * R + kSinkCells); i += NT)
+            x.reg_re[i] = 0.0;
     }
     const unsigned total = summary->total_items;
     for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
@@ -858,17 +898,10 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
             const int pass = 64 * pi.wpp * pi.nslot;
             for (int cb = 0; cb < pi.W * pi.W; cb += pass)
             {
-                switch (pi.nslot)
-                {
-                case 1: scatter_stream<T, DIAG, 1>(x, pi, wis, sa, sb, cb); break;
-                case 2: scatter_stream<T, DIAG, 2>(x, pi, wis, sa, sb, cb); break;
-                case 3: scatter_stream<T, DIAG, 3>(x, pi, wis, sa, sb, cb); break;
-                case 4: scatter_stream<T, DIAG, 4>(x, pi, wis, sa, sb, cb); break;
-                case 5: scatter_stream<T, DIAG, 5>(x, pi, wis, sa, sb, cb); break;
-                case 6: scatter_stream<T, DIAG, 6>(x, pi, wis, sa, sb, cb); break;
-                case 7: scatter_stream<T, DIAG, 7>(x, pi, wis, sa, sb, cb); break;
-                default: scatter_stream<T, DIAG, 8>(x, pi, wis, sa, sb, cb); break;
-                }
+                if (lds && !DIAG)
+                    scatter_dispatch<T, DIAG, true>(x, pi, wis, sa, sb, cb);
+                else
+                    scatter_dispatch<T, DIAG, false>(x, pi, wis, sa, sb, cb);
             }
         }
         __syncthreads();
@@ -1044,8 +1077,8 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     plane_bits_ = bits_plane_((int)P_);
 
     // Tile side T = 2^t (<= 64): the largest whose LDS region
-    // (T + Wmax - 1)^2, real and imaginary, fits the budget.  If none does
-    // the scatter adds straight into the grid in HBM.
+    // (T + Wmax - 1)^2 plus the sink cells, real and imaginary, fits the
+    // budget.  If none does the scatter adds straight into the grid in HBM.
     const size_t real = sizeof(double);   // the region is FP64
     const long long wmax = 2ll * s_max_ + 1;
     region_ = 0;
@@ -1053,19 +1086,22 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     for (int t = std::min(6, coord_bits_); t >= 0; --t)
     {
         const long long R = (1ll << t) + wmax - 1;
-        if ((size_t)(2 * R * R) * real <= (size_t)kRegionBytes)
+        if ((size_t)(2 * (R * R + kSinkCells)) * real <=
+                (size_t)kRegionBytes)
         {
             tile_bits_ = t;
             region_ = (int)R;
             break;
         }
     }
-    lds_bytes_ = (size_t)2 * region_ * region_ * real;
+    lds_bytes_ = region_ > 0 ? (size_t)2 *
+            ((size_t)region_ * region_ + kSinkCells) * real : 0;
     tile_shift_ = 2 * tile_bits_ + combo_bits_ + plane_bits_;
     key_bits_ = 2 * coord_bits_ + combo_bits_ + plane_bits_;
 
-    // Streams per plane: wpp waves cover the footprint with <= kSlotsMax
-    // cells per lane (larger footprints take several passes).
+    // Streams per plane: wpp waves cover the footprint with <= smax cells
+    // per lane (larger footprints take several passes).
+    const int smax = precision == 0x01 ? kSlotsMax : kSlotsMaxF64;
     pinfo_.resize(P_);
     for (size_t p = 0; p < P_; ++p)
     {
@@ -1075,9 +1111,9 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
         const long long cells = (long long)pi.W * pi.W;
         pi.wpp = 1;
         while (pi.wpp < kGridThreads / 64 &&
-                (long long)pi.wpp * 64 * kSlotsMax < cells)
+                (long long)pi.wpp * 64 * smax < cells)
             pi.wpp *= 2;
-        pi.nslot = (int)std::min<long long>(kSlotsMax,
+        pi.nslot = (int)std::min<long long>(smax,
                 (cells + 64ll * pi.wpp - 1) / (64ll * pi.wpp));
         pi.pad0 = pi.pad1 = 0;
         pi.eoff = 0;

@@ -13,15 +13,20 @@
 namespace gw {
 
 // Threads of a scatter workgroup (16 waves), most footprint cells one lane
-// owns at a time (registers per slot: tap and running sum, complex), and
-// the most points in one scatter job (the per-job plane segment table
+// owns at a time (registers per slot: current and next tap and running
+// sum, complex; FP64 slots take twice the registers, so half as many fit),
+// and the most points in one scatter job (the per-job plane segment table
 // lives in LDS).
 constexpr int kGridThreads = 1024;
 constexpr int kSlotsMax = 8;
+constexpr int kSlotsMaxF64 = 4;
 constexpr int kJobPoints = 2 * kGridThreads;
 // LDS bytes for the job's grid region (planar FP64 real and imaginary
 // arrays; 88 x 88 cells fit).
 constexpr int kRegionBytes = 124 * 1024;
+// Cells after each region array that lanes past a footprint add into (one
+// per lane of a wave; never read).
+constexpr int kSinkCells = 64;
 
 // Per-plane constants, fixed by the support table (host-computed at plan
 // creation, uploaded once).
