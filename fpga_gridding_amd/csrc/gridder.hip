@@ -6,36 +6,29 @@
 //
 //   k_locate   per point: the reference's index math, bit-exact
 //              (oskar_grid_wproj.c:169-212), skip count, per-(plane,
-//              sub-cell) histogram, footprint bounding box, sort key
-//              (plane, Morton(u0, v0)) and a 16/32-byte point record;
-//   sort       radix sort of the keys (rocPRIM): points grouped by plane and
-//              ordered along a Z-curve inside each plane;
+//              sub-cell) histogram, footprint bounding box, a sort key
+//              (tile of the footprint origin, plane, origin inside the
+//              tile, sub-cell table) and a 16/32-byte point record;
+//   sort       radix sort of the keys (rocPRIM);
 //   k_plan     points gridded, update count and norm increment (histogram x
-//              per-(plane, sub-cell) tap sums);
-//   k_gather   sorted point records (position, position mod W, sub-cell
-//              table, visibility pre-multiplied by the conjugation sign) and
-//              the heads of the tile buckets: runs of points of one plane
-//              whose footprint origins share an aligned T x T tile (T per
-//              plane, a power of two, so tiles are Z-curve runs);
-//   scans      bucket starts and work items (<= L points of one bucket);
-//   k_grid     the scatter, one work item per workgroup.  Lane slots own
-//              footprint-cell residues (Romein's scheme): slot (rx, ry)
-//              always feeds the grid cell congruent to (rx, ry) mod W inside
-//              the current footprint, so it sums in registers while
-//              consecutive points leave that cell in place.  When the
-//              footprint moves the cell elsewhere the slot adds its sum into
-//              an LDS copy of the bucket's (T + W - 1)^2 region -- a plain
-//              read-modify-write, since each region cell belongs to exactly
-//              one residue and so to one lane.  At the end of the item the
-//              region is added to the grid in HBM with coalesced atomics.
+//              per-(plane, sub-cell) tap sums, fixed summation order);
+//   k_gather   sorted point records (origin, sub-cell table, plane; the
+//              visibility pre-multiplied by the conjugation sign) and the
+//              heads of the tile buckets;
+//   scans      bucket starts and jobs (<= kJobPoints points of one tile);
+//   k_grid     the scatter, one job per workgroup, into an FP64 LDS copy of
+//              the tile's grid region, added to the grid in HBM once per job.
+
 #include "gridder_internal.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace gw {
 
@@ -201,8 +194,10 @@ __global__ void __launch_bounds__(256) k_locate(
 
     const int centre = G / 2;
     const T ovf = (T)ov;
-    // Sort key, most significant first: tile (Z-order), plane, sub-cell
-    // table, position inside the tile (Z-order).  Skipped points sort last.
+    // Sort key, most significant first: tile (Z-order), plane, position
+    // inside the tile (Z-order), sub-cell table.  Skipped points sort last.
+    // (Points at one position then share the scatter's running sums even
+    // when their sub-cell tables differ.)
     const unsigned long long skip_key = 1ull << key_bits;
     const int sh_combo = 2 * tile_bits;
     const int sh_plane = sh_combo + combo_bits;
@@ -255,8 +250,8 @@ __global__ void __launch_bounds__(256) k_locate(
                     (spread_bits16((unsigned)v0) << 1);
             key = ((mort >> sh_combo) << sh_tile) |
                     ((unsigned long long)plane << sh_plane) |
-                    ((unsigned long long)table << sh_combo) |
-                    (mort & ((1ull << sh_combo) - 1ull));
+                    ((mort & ((1ull << sh_combo) - 1ull)) << combo_bits) |
+                    (unsigned long long)table;
             if (lds_hist) atomicAdd(&s_combo[c], 1u);
             else atomicAdd(&combo[c], 1u);
             vmin = min(vmin, v0);
@@ -586,22 +581,39 @@ __global__ void __launch_bounds__(256) k_write_items(
 // -------------------------------------------------------------------------
 //
 // A job is up to kJobPoints sorted points of one T x T tile of footprint
-// origins, ordered by plane, then sub-cell table, then position.  The
+// origins, ordered by plane, then position, then sub-cell table.  The
 // workgroup accumulates them into an LDS copy of the tile's
 // (T + Wmax - 1)^2 grid region (planar real / imaginary arrays), across
 // planes, and adds the region to the grid in HBM once at the end of the job.
 //
 // Inside a plane segment the waves form streams of wpp waves; each stream
-// takes a contiguous share of the segment's points.  Lane slot s of wave w
-// (w-th in its stream) owns footprint cell c = w*64 + lane + s*64*wpp,
+// takes a contiguous chunk of the segment's points (at least min_share
+// points; chunk q goes to stream (q + first point of the segment) mod
+// streams, so short segments spread over all streams).  Lane slot s of wave
+// w (w-th in its stream) owns footprint cell c = w*64 + lane + s*64*wpp,
 // i.e. offset (j, k) = (c / W, c mod W) from the footprint origin, whose tap
 // for the point's sub-cell table is etab[table][c]
 // (= conv_func[p][|off_v + (j-S)*ov|][|off_u + (k-S)*ov|]).  The tap stays in
-// registers while consecutive points share the plane and table; the product
-// with the visibility stays in registers while they share the position;
-// when the position changes the slot adds its sum to the region with an LDS
-// atomic.  For planes whose region would not fit in LDS the sums go straight
-// to the grid in HBM.
+// registers while consecutive points share the plane and table; the products
+// with the visibilities are summed in registers while consecutive points
+// share the position (whatever their tables); when the position changes the
+// slot adds its sum to the region with an LDS atomic.  For planes whose
+// region would not fit in LDS the sums go straight to the grid in HBM.
+
+// LDS of a scatter workgroup.  The imaginary array sits at a fixed byte
+// offset (kImBytes < 64 KiB) from the real one, so both atomics of a cell
+// share one address register and differ only in the instruction's offset.
+struct GridLds
+{
+    double re[kRegionCells];      // region real parts, then the sink cells
+    double im[kRegionCells];      // region imaginary parts, then the sinks
+    unsigned seg[kJobPoints + 1]; // plane segment starts within the job
+    int segp[kJobPoints];         // plane of each segment
+    unsigned tmp[17];
+    unsigned nseg;
+};
+constexpr unsigned kImBytes = kRegionCells * sizeof(double);
+static_assert(kImBytes < 65536, "imaginary offset must fit a DS offset");
 
 template <typename T>
 struct ScatterCtx
@@ -611,29 +623,51 @@ struct ScatterCtx
     const int4* __restrict__ meta;
     const T4* __restrict__ vq;
     const T2* __restrict__ etab;
-    const PlaneInfo* __restrict__ pinfo;
-    double* reg_re;                // LDS region, real parts (FP64)
-    double* reg_im;                // LDS region, imaginary parts (FP64)
+    GridLds* lds;
     T* __restrict__ grid;
     unsigned* __restrict__ diag_count;
     unsigned long long* __restrict__ diag_wsum;
     int R, G, row0, ux0, vy0;
-    bool lds;
 };
 
 // The LDS region is FP64: on gfx950 ds_add_f64 sustains about 20 times the
 // throughput of ds_add_f32 (measured: ~7 vs ~0.33 lane-ops per clock per
 // CU), and the tile's sums are then carried in double until the one add
-// into the grid per job.
-__device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
+// into the grid per job.  c8 is the cell's byte offset in the real array.
+__device__ __forceinline__ void lds_add_cell(GridLds* L, unsigned c8,
+                                             double re, double im)
+{
+    char* const b = reinterpret_cast<char*>(L->re);
+    atomicAdd(reinterpret_cast<double*>(b + c8), re);
+    atomicAdd(reinterpret_cast<double*>(b + kImBytes + c8), im);
+}
+
+// Makes the compiler complete the load that produced v before this point:
+// an empty asm that reads and rewrites the register.  It is not volatile
+// and touches no memory, so loads of read-only data around it can still be
+// scalar loads.
+template <typename T>
+__device__ __forceinline__ void settle(T& v)
+{
+    asm("" : "+v"(v));
+}
 
 // One stream's share [a, b) of a plane segment: footprint cells
 // [cb, cb + 64 * wpp * NS), NS cells per lane.  LDS selects the LDS region
-// (else the grid in HBM).  Point records are read two points ahead and the
-// next sub-cell table's taps are loaded before the current point's
-// arithmetic, so both latencies overlap work.  In the LDS path lanes past
-// the footprint are not masked off: their sums go to a per-lane sink cell
-// after the region (kSinkCells of them), which is never read.
+// (else the grid in HBM).
+//
+// Arithmetic is FP64 from the products on (gfx950 issues FP64 FMA at the
+// rate of unpacked FP32 FMA): the current taps are held as double, a
+// product of two floats is exact in double, and the sums need no
+// conversion on their way into the FP64 region.  The first point of a
+// position run writes its products with a multiply, so a flush leaves the
+// sums as they are instead of clearing them.
+//
+// Point records are read two points ahead and the next sub-cell table's
+// taps are loaded before the current point's arithmetic, so both latencies
+// overlap work.  In the LDS path lanes past the footprint are not masked
+// off: their sums go to a per-lane sink cell after the region (kSinkCells
+// of them), which is never read.
 template <typename T, bool DIAG, bool LDS, int NS>
 __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
         const PlaneInfo& pi, int wis, int a, int b, int cb)
@@ -648,70 +682,103 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
     const size_t table_bytes = (size_t)W2 * sizeof(T2);
     const int cstride = LDS ? x.R : x.G;
     unsigned toff[NS];     // byte offset of the lane's tap in a table
-    unsigned loff[NS];     // cell offset from the footprint origin
+    unsigned loff[NS];     // LDS: byte offset from the footprint origin in
+                           // the region; HBM: cell offset in the grid
     bool valid[NS];
-    T tre[NS], tim[NS], nre[NS], nim[NS];
-    T acc_re[NS], acc_im[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
+    typedef typename std::conditional<kSumF64, double, T>::type A;
+    A tre[NS], tim[NS];        // taps of the current table
+    T nre[NS], nim[NS];        // taps of the next table, in flight
+    A acc_re[NS], acc_im[NS];
+    const int c0 = cb + wis * 64 + lane;
+    if (pi.wmagic != 0u)
     {
-        const int c = cb + wis * 64 + lane + s * stride;
-        valid[s] = c < W2;
-        const int cc = valid[s] ? c : 0;
-        const int j = cc / W;
-        const int k = cc - j * W;
-        loff[s] = (unsigned)(j * cstride + k);
-        toff[s] = (unsigned)cc * (unsigned)sizeof(T2);
-        acc_re[s] = (T)0;
-        acc_im[s] = (T)0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            const int c = c0 + s * stride;
+            valid[s] = c < W2;
+            const unsigned cc = valid[s] ? (unsigned)c : 0u;
+            const unsigned j = __umulhi(cc, pi.wmagic);
+            const unsigned k = cc - j * (unsigned)W;
+            loff[s] = j * (unsigned)cstride + k;
+            toff[s] = cc * (unsigned)sizeof(T2);
+        }
     }
-    auto load_taps = [&](int table, T* re, T* im) {
+    else
+    {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            const int c = c0 + s * stride;
+            valid[s] = c < W2;
+            const unsigned cc = valid[s] ? (unsigned)c : 0u;
+            const unsigned j = cc / (unsigned)W;
+            const unsigned k = cc - j * (unsigned)W;
+            loff[s] = j * (unsigned)cstride + k;
+            toff[s] = cc * (unsigned)sizeof(T2);
+        }
+    }
+    auto load_taps = [&](int table) {
         const char* tp = ptab + (size_t)table * table_bytes;
 #pragma unroll
         for (int s = 0; s < NS; ++s)
         {
             const T2 t = *reinterpret_cast<const T2*>(tp + toff[s]);
-            re[s] = t.x;
-            im[s] = t.y;
+            nre[s] = t.x;
+            nim[s] = t.y;
         }
     };
+    auto take_taps = [&]() {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            tre[s] = (A)nre[s];
+            tim[s] = (A)nim[s];
+        }
+    };
+    int4 m = x.meta[a];
+    T4 v = x.vq[a];
+    load_taps(m.y);
+    if (LDS)
+    {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) loff[s] *= (unsigned)sizeof(double);
+    }
     auto origin = [&](int pos) -> unsigned {
         const int u0 = pos & 0xFFFF;
         const int v0 = (int)((unsigned)pos >> 16);
-        return LDS ? (unsigned)((v0 - x.vy0) * x.R + (u0 - x.ux0)) :
+        return LDS ? (unsigned)((v0 - x.vy0) * x.R + (u0 - x.ux0)) *
+                        (unsigned)sizeof(double) :
                 (unsigned)((v0 - x.row0) * x.G + u0);
     };
     unsigned cnt = 0;
     unsigned long long wsum = 0;
-    const unsigned sink = (unsigned)(x.R * x.R + lane);
+    const unsigned sink8 = (unsigned)(x.R * x.R + lane) *
+            (unsigned)sizeof(double);
     auto flush = [&](unsigned base) {
 #pragma unroll
         for (int s = 0; s < NS; ++s)
         {
             if (LDS)
             {
-                const unsigned c = valid[s] ? base + loff[s] : sink;
-                lds_add(x.reg_re + c, (double)acc_re[s]);
-                lds_add(x.reg_im + c, (double)acc_im[s]);
+                const unsigned c8 = valid[s] ? base + loff[s] : sink8;
+                lds_add_cell(x.lds, c8, (double)acc_re[s],
+                             (double)acc_im[s]);
             }
             else if (valid[s])
             {
                 const size_t g = (size_t)base + loff[s];
-                atomic_add_real(x.grid + 2 * g, acc_re[s]);
-                atomic_add_real(x.grid + 2 * g + 1, acc_im[s]);
+                atomic_add_real(x.grid + 2 * g, (T)acc_re[s]);
+                atomic_add_real(x.grid + 2 * g + 1, (T)acc_im[s]);
                 if (DIAG)
                 {
                     atomicAdd(&x.diag_count[g], cnt);
                     atomicAdd(&x.diag_wsum[g], wsum);
                 }
             }
-            acc_re[s] = (T)0;
-            acc_im[s] = (T)0;
         }
     };
 
-    int4 m = x.meta[a];
-    T4 v = x.vq[a];
     int4 mn = m;
     T4 vn = v;
     if (a + 1 < b)
@@ -719,8 +786,11 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
         mn = x.meta[a + 1];
         vn = x.vq[a + 1];
     }
-    load_taps(m.y, tre, tim);
     unsigned base = origin(m.x);
+    // The first table's taps are converted before the loop, so the loads
+    // issued inside it stay in flight across a point's work.
+    take_taps();
+    bool fresh = true;
     for (int k = a; k < b; ++k)
     {
         const bool more = k + 1 < b;
@@ -732,15 +802,30 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
             vnn = x.vq[k + 2];
         }
         const bool new_table = more && mn.y != m.y;
-        if (new_table) load_taps(mn.y, nre, nim);
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
+        if (new_table) load_taps(mn.y);
+        const A vx = (A)v.x, vy = (A)v.y;
+        const A vz = (A)v.z, vw = (A)v.w;
+        if (fresh)
         {
-            acc_re[s] = fma(v.x, tre[s], acc_re[s]);
-            acc_re[s] = fma(v.y, tim[s], acc_re[s]);
-            acc_im[s] = fma(v.z, tre[s], acc_im[s]);
-            acc_im[s] = fma(v.w, tim[s], acc_im[s]);
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+            {
+                acc_re[s] = fma(vy, tim[s], vx * tre[s]);
+                acc_im[s] = fma(vw, tim[s], vz * tre[s]);
+            }
         }
+        else
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+            {
+                acc_re[s] = fma(vx, tre[s], acc_re[s]);
+                acc_re[s] = fma(vy, tim[s], acc_re[s]);
+                acc_im[s] = fma(vz, tre[s], acc_im[s]);
+                acc_im[s] = fma(vw, tim[s], acc_im[s]);
+            }
+        }
+        fresh = false;
         if (DIAG)
         {
             cnt += 1;
@@ -751,18 +836,11 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
         {
             flush(base);
             base = origin(mn.x);
+            fresh = true;
             cnt = 0;
             wsum = 0;
         }
-        if (new_table)
-        {
-#pragma unroll
-            for (int s = 0; s < NS; ++s)
-            {
-                tre[s] = nre[s];
-                tim[s] = nim[s];
-            }
-        }
+        if (new_table) take_taps();
         m = mn;
         v = vn;
         mn = mnn;
@@ -771,68 +849,63 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
     flush(base);
 }
 
+template <typename T> constexpr int slots_upto(int n)
+{
+    return n < slots_max<T>() ? n : slots_max<T>();
+}
+
 template <typename T, bool DIAG, bool LDS>
 __device__ __forceinline__ void scatter_dispatch(const ScatterCtx<T>& x,
         const PlaneInfo& pi, int wis, int a, int b, int cb)
 {
-    // FP64 plans never set nslot above kSlotsMaxF64 (the plan caps it), so
-    // the wider cases are only compiled for FP32.
-    constexpr bool wide = sizeof(T) == 4;
+    // The plan never sets nslot above slots_max<T>() (<= 8).
+    static_assert(slots_max<T>() <= 8, "dispatch covers 1..8 slots");
     switch (pi.nslot)
     {
     case 1: scatter_stream<T, DIAG, LDS, 1>(x, pi, wis, a, b, cb); break;
-    case 2: scatter_stream<T, DIAG, LDS, 2>(x, pi, wis, a, b, cb); break;
-    case 3: scatter_stream<T, DIAG, LDS, 3>(x, pi, wis, a, b, cb); break;
-    case 4: scatter_stream<T, DIAG, LDS, 4>(x, pi, wis, a, b, cb); break;
-    case 5: scatter_stream<T, DIAG, LDS, wide ? 5 : 4>(x, pi, wis, a, b, cb);
-        break;
-    case 6: scatter_stream<T, DIAG, LDS, wide ? 6 : 4>(x, pi, wis, a, b, cb);
-        break;
-    case 7: scatter_stream<T, DIAG, LDS, wide ? 7 : 4>(x, pi, wis, a, b, cb);
-        break;
-    default: scatter_stream<T, DIAG, LDS, wide ? 8 : 4>(x, pi, wis, a, b, cb);
-        break;
+    case 2: scatter_stream<T, DIAG, LDS, slots_upto<T>(2)>(x, pi, wis, a, b, cb); break;
+    case 3: scatter_stream<T, DIAG, LDS, slots_upto<T>(3)>(x, pi, wis, a, b, cb); break;
+    case 4: scatter_stream<T, DIAG, LDS, slots_upto<T>(4)>(x, pi, wis, a, b, cb); break;
+    case 5: scatter_stream<T, DIAG, LDS, slots_upto<T>(5)>(x, pi, wis, a, b, cb); break;
+    case 6: scatter_stream<T, DIAG, LDS, slots_upto<T>(6)>(x, pi, wis, a, b, cb); break;
+    case 7: scatter_stream<T, DIAG, LDS, slots_upto<T>(7)>(x, pi, wis, a, b, cb); break;
+    default: scatter_stream<T, DIAG, LDS, slots_upto<T>(8)>(x, pi, wis, a, b, cb); break;
     }
 }
 
+// Grid-stride loop over the jobs (persistent: the plan launches as many
+// workgroups as fit the device at once).
 template <typename T, bool DIAG>
 __global__ void __launch_bounds__(kGridThreads) k_grid(
         const Summary* __restrict__ summary, const int4* __restrict__ items,
         const PlaneInfo* __restrict__ pinfo, const int4* __restrict__ meta,
         const typename Types<T>::T4* __restrict__ vq,
         const typename Types<T>::T2* __restrict__ etab, int G, int row0,
-        int R, T* __restrict__ grid, int force_direct,
+        int R, T* __restrict__ grid, int force_direct, int min_share,
         unsigned* __restrict__ diag_count,
         unsigned long long* __restrict__ diag_wsum)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ unsigned s_seg[kJobPoints + 1];   // segment starts
-    __shared__ int s_segp[kJobPoints];           // segment planes
-    __shared__ unsigned s_tmp[17];
-    __shared__ unsigned s_nseg;
+    __shared__ GridLds L;
     const int tid = threadIdx.x;
     const int NT = blockDim.x;
-    const int nwaves = NT >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool lds = !force_direct && R > 0;
     ScatterCtx<T> x;
     x.meta = meta;
     x.vq = vq;
     x.etab = etab;
-    x.pinfo = pinfo;
-    x.reg_re = reinterpret_cast<double*>(lds_raw);
-    x.reg_im = x.reg_re + (lds ? R * R + kSinkCells : 0);
+    x.lds = &L;
     x.grid = grid;
     x.diag_count = diag_count;
     x.diag_wsum = diag_wsum;
     x.R = R;
     x.G = G;
     x.row0 = row0;
-    x.lds = lds;
-    if (lds)
+    const int rcells = lds ? R * R + kSinkCells : 0;
+    for (int i = tid; i < rcells; i += NT)
     {
-        for (int i = tid; i < 2 * (R * R + kSinkCells); i += NT)
-            x.reg_re[i] = 0.0;
+        L.re[i] = 0.0;
+        L.im[i] = 0.0;
     }
     const unsigned total = summary->total_items;
     for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
@@ -858,42 +931,43 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
             }
         }
         unsigned nseg;
-        unsigned pos = block_exclusive_scan(heads, s_tmp, &nseg);
+        unsigned pos = block_exclusive_scan(heads, L.tmp, &nseg);
 #pragma unroll
         for (int q = 0; q < 2; ++q)
         {
             if (myhead[q])
             {
-                s_seg[pos] = (unsigned)(2 * tid + q);
-                s_segp[pos] = myplane[q];
+                L.seg[pos] = (unsigned)(2 * tid + q);
+                L.segp[pos] = myplane[q];
                 ++pos;
             }
         }
         if (tid == 0)
         {
-            s_seg[nseg] = (unsigned)len;
-            s_nseg = nseg;
+            L.seg[nseg] = (unsigned)len;
+            L.nseg = nseg;
         }
         __syncthreads();
         // Values read back from LDS are the same in every lane; make that
         // explicit so that the point loop runs on scalar registers and
         // scalar loads.
-        nseg = (unsigned)__builtin_amdgcn_readfirstlane((int)s_nseg);
+        nseg = (unsigned)__builtin_amdgcn_readfirstlane((int)L.nseg);
         for (unsigned g = 0; g < nseg; ++g)
         {
-            const int plane = __builtin_amdgcn_readfirstlane(s_segp[g]);
+            const int plane = __builtin_amdgcn_readfirstlane(L.segp[g]);
             const PlaneInfo pi = pinfo[plane];
-            const int a = start +
-                    __builtin_amdgcn_readfirstlane((int)s_seg[g]);
-            const int b = start +
-                    __builtin_amdgcn_readfirstlane((int)s_seg[g + 1]);
-            const int streams = nwaves / pi.wpp;
+            const int ra = __builtin_amdgcn_readfirstlane((int)L.seg[g]);
+            const int rb = __builtin_amdgcn_readfirstlane((int)L.seg[g + 1]);
+            // streams = nwaves / wpp, a power of two (2^sshift).
+            const int smask = (1 << pi.sshift) - 1;
             const int stream = wave / pi.wpp;
+            if (stream > smask) continue;
             const int wis = wave - stream * pi.wpp;
-            if (stream >= streams) continue;
-            const int share = (b - a + streams - 1) / streams;
-            const int sa = a + stream * share;
-            const int sb = min(b, sa + share);
+            const int n = rb - ra;
+            const int share = max(min_share, (n + smask) >> pi.sshift);
+            const int q = (stream - ra) & smask;
+            const int sa = start + ra + q * share;
+            const int sb = min(start + rb, sa + share);
             if (sa >= sb) continue;
             const int pass = 64 * pi.wpp * pi.nslot;
             for (int cb = 0; cb < pi.W * pi.W; cb += pass)
@@ -912,12 +986,12 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
             // (they may lie off the grid).
             for (int i = tid; i < R * R; i += NT)
             {
-                const double re = x.reg_re[i];
-                const double im = x.reg_im[i];
+                const double re = L.re[i];
+                const double im = L.im[i];
                 if (re != 0.0 || im != 0.0)
                 {
-                    x.reg_re[i] = 0.0;
-                    x.reg_im[i] = 0.0;
+                    L.re[i] = 0.0;
+                    L.im[i] = 0.0;
                     const int y = i / R;
                     const int xx = i - y * R;
                     const size_t g = (size_t)(x.vy0 - row0 + y) * (size_t)G +
@@ -1079,29 +1153,26 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     // Tile side T = 2^t (<= 64): the largest whose LDS region
     // (T + Wmax - 1)^2 plus the sink cells, real and imaginary, fits the
     // budget.  If none does the scatter adds straight into the grid in HBM.
-    const size_t real = sizeof(double);   // the region is FP64
     const long long wmax = 2ll * s_max_ + 1;
     region_ = 0;
     tile_bits_ = std::min(6, coord_bits_);
     for (int t = std::min(6, coord_bits_); t >= 0; --t)
     {
         const long long R = (1ll << t) + wmax - 1;
-        if ((size_t)(2 * (R * R + kSinkCells)) * real <=
-                (size_t)kRegionBytes)
+        if (R * R + kSinkCells <= kRegionCells)
         {
             tile_bits_ = t;
             region_ = (int)R;
             break;
         }
     }
-    lds_bytes_ = region_ > 0 ? (size_t)2 *
-            ((size_t)region_ * region_ + kSinkCells) * real : 0;
     tile_shift_ = 2 * tile_bits_ + combo_bits_ + plane_bits_;
     key_bits_ = 2 * coord_bits_ + combo_bits_ + plane_bits_;
 
     // Streams per plane: wpp waves cover the footprint with <= smax cells
     // per lane (larger footprints take several passes).
-    const int smax = precision == 0x01 ? kSlotsMax : kSlotsMaxF64;
+    const int smax = precision == 0x01 ? slots_max<float>() :
+            slots_max<double>();
     pinfo_.resize(P_);
     for (size_t p = 0; p < P_; ++p)
     {
@@ -1115,8 +1186,25 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
             pi.wpp *= 2;
         pi.nslot = (int)std::min<long long>(smax,
                 (cells + 64ll * pi.wpp - 1) / (64ll * pi.wpp));
-        pi.pad0 = pi.pad1 = 0;
+        pi.sshift = 0;
+        while ((kGridThreads / 64 / pi.wpp) >> (pi.sshift + 1)) ++pi.sshift;
+        // ceil(2^32 / W) divides exactly over c < W^2 when
+        // (W^2 - 1) * (m * W - 2^32) < 2^32 (W == 1: only c == 0 occurs).
+        const unsigned long long two32 = 1ull << 32;
+        const unsigned long long wm = pi.W == 1 ? 0xFFFFFFFFull :
+                (two32 + (unsigned long long)pi.W - 1) / (unsigned long long)pi.W;
+        const unsigned long long err = wm * (unsigned long long)pi.W - two32;
+        pi.wmagic = (pi.W == 1 ||
+                (unsigned long long)(cells - 1) * err < two32) ?
+                (unsigned)wm : 0u;
         pi.eoff = 0;
+    }
+
+    // Tuning override for sweeps: GW_SCATTER_MIN_SHARE=<1..2048>.
+    if (const char* e = std::getenv("GW_SCATTER_MIN_SHARE"))
+    {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= kJobPoints) min_share_ = v;
     }
 
     check_hip(hipGetDevice(&device_), kErrComputeDevices, "hipGetDevice");
@@ -1126,7 +1214,7 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     int occ = 0;
     check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ,
             precision == 0x01 ? (const void*)k_grid<float, false> :
-            (const void*)k_grid<double, false>, kGridThreads, lds_bytes_),
+            (const void*)k_grid<double, false>, kGridThreads, 0),
             kErrComputeDevices, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     grid_blocks_ = n_cu_ * std::max(1, occ);
     check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
@@ -1375,14 +1463,15 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
                 (const Summary*)d_summary_, (const int4*)d_items_,
                 (const PlaneInfo*)d_pinfo_, (const int4*)d_meta_,
                 (const T4*)d_vq_, (const T2*)d_etab_, G_, row0, 0, grid, 1,
-                diag->per_cell, diag->wsum_cell);
+                min_share_, diag->per_cell, diag->wsum_cell);
     else
         hipLaunchKernelGGL((k_grid<T, false>), dim3(grid_blocks_),
-                dim3(kGridThreads), lds_bytes_, stream_,
+                dim3(kGridThreads), 0, stream_,
                 (const Summary*)d_summary_, (const int4*)d_items_,
                 (const PlaneInfo*)d_pinfo_, (const int4*)d_meta_,
                 (const T4*)d_vq_, (const T2*)d_etab_, G_, row0, region_, grid,
-                0, (unsigned*)nullptr, (unsigned long long*)nullptr);
+                0, min_share_, (unsigned*)nullptr,
+                (unsigned long long*)nullptr);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_grid launch");
     check_hip(hipEventRecord(ev_[4], stream_), kErrKernelLaunch, "event");
 }

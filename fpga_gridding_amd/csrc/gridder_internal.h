@@ -13,20 +13,28 @@
 namespace gw {
 
 // Threads of a scatter workgroup (16 waves), most footprint cells one lane
-// owns at a time (registers per slot: current and next tap and running
-// sum, complex; FP64 slots take twice the registers, so half as many fit),
-// and the most points in one scatter job (the per-job plane segment table
-// lives in LDS).
+// owns at a time (registers per slot: the current taps and the running
+// sums in FP64, the next taps in the plan's precision, complex; 6 slots
+// keep a wave within 128 VGPRs), and the most points in one scatter job
+// (the per-job plane segment table lives in LDS).
 constexpr int kGridThreads = 1024;
-constexpr int kSlotsMax = 8;
-constexpr int kSlotsMaxF64 = 4;
+constexpr bool kSumF64 = false;
+template <typename T> constexpr int slots_max()
+{
+    return (kSumF64 || sizeof(T) == 8) ? 6 : 8;
+}
 constexpr int kJobPoints = 2 * kGridThreads;
-// LDS bytes for the job's grid region (planar FP64 real and imaginary
-// arrays; 88 x 88 cells fit).
+// LDS bytes for the job's grid region: planar FP64 real and imaginary
+// arrays of kRegionCells each, holding the R x R region and then the sink
+// cells (88 x 88 cells fit).
 constexpr int kRegionBytes = 124 * 1024;
+constexpr int kRegionCells = kRegionBytes / 16;
 // Cells after each region array that lanes past a footprint add into (one
 // per lane of a wave; never read).
 constexpr int kSinkCells = 64;
+// Default least number of points a scatter stream takes from a plane
+// segment (shorter chunks pay the stream's set-up for too little work).
+constexpr int kMinShare = 8;
 
 // Per-plane constants, fixed by the support table (host-computed at plan
 // creation, uploaded once).
@@ -36,7 +44,10 @@ struct PlaneInfo
     int S;       // support
     int wpp;     // waves per stream (1, 2, 4, 8 or 16)
     int nslot;   // footprint cells per lane per pass (<= kSlotsMax)
-    int pad0, pad1;
+    // c / W == umulhi(c, wmagic) for every cell index c < W*W, or 0 if no
+    // 32-bit multiplier is exact over that range (then plain division).
+    unsigned wmagic;
+    int sshift;  // log2 of the streams per workgroup, kGridThreads/64/wpp
     // First tap of this plane in the expanded tap table: K*K sub-cell
     // tables of W*W taps each, tap (j, k) at [(iv*K + iu)*W*W + j*W + k].
     unsigned long long eoff;
@@ -163,7 +174,7 @@ private:
     int region_;         // LDS region side T + Wmax - 1 (0: none)
     std::vector<int> support_;
     std::vector<PlaneInfo> pinfo_;
-    size_t lds_bytes_ = 0;     // dynamic LDS of the scatter kernel
+    int min_share_ = kMinShare; // least points per scatter stream
     int device_ = 0;
     int n_cu_ = 0;
     int grid_blocks_ = 0;
