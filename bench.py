@@ -6,6 +6,7 @@ into an 8192^2 grid, with points, visibilities and the grid resident in HBM
 (the plan API: gw_grid_device + gw_plan_results).  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  python bench.py --copy-bytes B     (profiler calibration: one plain copy)
 
 The scatter kernel's duration comes from HIP events the library records on
 its own stream around each stage; the roofline uses the algorithmic FLOPs of
@@ -51,6 +52,32 @@ def _traffic_from_profile():
             except (IndexError, ValueError):
                 return None
     return None
+
+
+def accuracy_sample(wl, n_sample=200_000, seed=4):
+    """Errors of the HIP gridder on a random sample of the workload against
+    the FP64 oracle (oskar_grid_wproj_d semantics: FP64 index math and sums)
+    and against the exact oracle (FP32 index math as oskar_grid_wproj_f,
+    FP64 sums): max|d|/max|g| and RMS(d)/RMS(g)."""
+    import oracle
+    import fpga_gridding_amd as fg
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(wl.num_points, n_sample, replace=False))
+    vis = wl.vis.reshape(-1, 2)[idx].reshape(-1)
+    sub = dataclasses.replace(wl, uu=np.ascontiguousarray(wl.uu[idx]),
+                              vv=np.ascontiguousarray(wl.vv[idx]),
+                              ww=np.ascontiguousarray(wl.ww[idx]),
+                              vis=np.ascontiguousarray(vis))
+    g, _, _ = fg.grid_workload_f(sub)
+    out = {"sample": f"{n_sample} random points of the workload"}
+    for name, fn in (("fp64_oracle", oracle.grid_wproj_d),
+                     ("exact_oracle", oracle.grid_wproj_f_exact)):
+        ref = fn(sub)[0].astype(np.float64)
+        d = g.astype(np.float64) - ref
+        out[name] = {"max_rel": float(np.abs(d).max() / np.abs(ref).max()),
+                     "rms_rel": float(np.sqrt(np.mean(d * d)) /
+                                      np.sqrt(np.mean(ref * ref)))}
+    return out
 
 
 def cpu_baseline(wl, budget_s=12.0, seed=3):
@@ -120,7 +147,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true",
-                    help="skip the CPU baseline")
+                    help="skip the CPU baseline and the oracle error sample")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="profiler calibration: copy this many bytes once "
+                         "with the library's k_copy kernel and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not "
@@ -130,6 +160,14 @@ def main():
 
     import fpga_gridding_amd as fg
     from fpga_gridding_amd import synth
+
+    if args.copy_bytes > 0:
+        src = fg.DeviceArray(args.copy_bytes)
+        dst = fg.DeviceArray(args.copy_bytes)
+        src.zero()
+        dst.copy_from(src)
+        print(json.dumps({"copy_bytes": args.copy_bytes, "kernel": "k_copy"}))
+        return
 
     wl = synth.config(1)
     plan = fg.Plan.for_workload(wl)
@@ -209,6 +247,7 @@ def main():
         out["cpu_baseline"] = {k: cb[k] for k in
                                ("value", "unit", "cores", "kind", "sample")}
         out["cpu_baseline"]["grid_updates_per_s"] = cb["updates_per_s"]
+        out["accuracy"] = accuracy_sample(wl)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -34,7 +34,7 @@ EXPORTED_SYMBOLS = (
     "gw_plan_results", "gw_plan_timings",
     "gw_device_count", "gw_device_malloc", "gw_device_free",
     "gw_memcpy_h2d", "gw_memcpy_d2h", "gw_memset_device",
-    "gw_device_synchronize",
+    "gw_device_synchronize", "gw_device_copy",
 )
 
 
@@ -121,6 +121,8 @@ def lib():
     L.gw_memset_device.restype = None
     L.gw_device_synchronize.argtypes = [_ip]
     L.gw_device_synchronize.restype = None
+    L.gw_device_copy.argtypes = [_vp, _vp, _sz, _ip]
+    L.gw_device_copy.restype = None
     _lib = L
     return L
 
@@ -275,6 +277,15 @@ class DeviceArray:
                             ct.byref(st))
         _check(st, "gw_memcpy_d2h")
         return out
+
+    def copy_from(self, src: "DeviceArray", nbytes: int = None):
+        """Device-to-device copy of nbytes (default: all of src)."""
+        n = src.nbytes if nbytes is None else nbytes
+        if n > self.nbytes or n > src.nbytes:
+            raise ValueError("copy larger than a buffer")
+        st = ct.c_int(0)
+        lib().gw_device_copy(self.ptr, src.ptr, n, ct.byref(st))
+        _check(st, "gw_device_copy")
 
     def zero(self):
         st = ct.c_int(0)

@@ -579,6 +579,15 @@ GW_EXPORT void gw_memset_device(void* ptr, int value, size_t bytes,
     });
 }
 
+GW_EXPORT void gw_device_copy(void* dst, const void* src, size_t bytes,
+        int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] { gw::device_copy(dst, src, bytes); });
+}
+
 GW_EXPORT void gw_device_synchronize(int* status)
 {
     int local = 0;

@@ -1006,6 +1006,46 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
 }
 
 // -------------------------------------------------------------------------
+// Streaming copy (profiler calibration and plain use)
+// -------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_copy(uint4* __restrict__ dst,
+        const uint4* __restrict__ src, size_t n16, unsigned char* dst_tail,
+        const unsigned char* src_tail, int tail)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += stride)
+        dst[i] = src[i];
+    if (blockIdx.x == 0 && (int)threadIdx.x < tail)
+        dst_tail[threadIdx.x] = src_tail[threadIdx.x];
+}
+
+void device_copy(void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return;
+    if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src))
+            & 15u)
+        fail(kErrInvalidArgument, "gw_device_copy needs 16-byte aligned "
+             "pointers");
+    const size_t n16 = bytes / 16;
+    const int tail = (int)(bytes - n16 * 16);
+    int dev = 0, ncu = 0;
+    check_hip(hipGetDevice(&dev), kErrComputeDevices, "hipGetDevice");
+    check_hip(hipDeviceGetAttribute(&ncu,
+            hipDeviceAttributeMultiprocessorCount, dev), kErrComputeDevices,
+            "hipDeviceGetAttribute");
+    const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
+            (n16 + 255) / 256, (size_t)ncu * 8));
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, nullptr,
+            static_cast<uint4*>(dst), static_cast<const uint4*>(src), n16,
+            static_cast<unsigned char*>(dst) + n16 * 16,
+            static_cast<const unsigned char*>(src) + n16 * 16, tail);
+    check_hip(hipGetLastError(), kErrKernelLaunch, "k_copy launch");
+    check_hip(hipDeviceSynchronize(), kErrKernelLaunch, "k_copy");
+}
+
+// -------------------------------------------------------------------------
 // Plan
 // -------------------------------------------------------------------------
 namespace {

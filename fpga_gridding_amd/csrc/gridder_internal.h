@@ -108,6 +108,9 @@ struct Error
 void check_hip(hipError_t e, int code, const char* what);
 
 // Host-side argument checks of a gridding call (no device access): raises
+// Streaming device-to-device copy (k_copy) on the null stream; waits.
+void device_copy(void* dst, const void* src, size_t bytes);
+
 // kErrInvalidArgument for sizes the gridder cannot handle, including a
 // kernel cube too small for support and oversample.
 void validate_args(size_t num_w_planes, const int* support, int oversample,

@@ -146,6 +146,12 @@ GW_EXPORT void  gw_memcpy_d2h(void* dst, const void* src, size_t bytes,
 GW_EXPORT void  gw_memset_device(void* ptr, int value, size_t bytes,
         int* status);
 GW_EXPORT void  gw_device_synchronize(int* status);
+/* Device-to-device copy with the library's own streaming kernel (k_copy,
+ * 16-byte loads and stores); synchronous.  Besides plain use it calibrates
+ * the HBM byte counters of a profiler: a copy of B bytes moves exactly B
+ * bytes each way. */
+GW_EXPORT void  gw_device_copy(void* dst, const void* src, size_t bytes,
+        int* status);
 
 #ifdef __cplusplus
 }
