@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(HERE, "libfpga_gridding_amd.so")
 
 GW_SINGLE = 0x01
 GW_DOUBLE = 0x02
+GW_HALF = 0x03          # FP16 storage, FP32 products, compensated grid
+GW_BFLOAT16 = 0x04      # BF16 storage, FP32 products, compensated grid
 
 _lib = None
 
@@ -34,7 +36,7 @@ EXPORTED_SYMBOLS = (
     "gw_plan_results", "gw_plan_timings",
     "gw_device_count", "gw_device_malloc", "gw_device_free",
     "gw_memcpy_h2d", "gw_memcpy_d2h", "gw_memset_device",
-    "gw_device_synchronize", "gw_device_copy",
+    "gw_device_synchronize", "gw_device_copy", "gw_compensated_resolve",
 )
 
 
@@ -123,6 +125,8 @@ def lib():
     L.gw_device_synchronize.restype = None
     L.gw_device_copy.argtypes = [_vp, _vp, _sz, _ip]
     L.gw_device_copy.restype = None
+    L.gw_compensated_resolve.argtypes = [_vp, _sz, _vp, _vp, _ip]
+    L.gw_compensated_resolve.restype = None
     _lib = L
     return L
 
@@ -134,6 +138,70 @@ def _check(status: ct.c_int, where: str):
 
 def _c(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
+
+
+def to_storage(a, precision):
+    """float32 values as the 16-bit storage of a GW_HALF (numpy float16) or
+    GW_BFLOAT16 (uint16 holding the top half of the FP32, rounded to
+    nearest even) plan."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if precision == GW_HALF:
+        return a.astype(np.float16)
+    if precision == GW_BFLOAT16:
+        u = a.view(np.uint32).astype(np.uint64)
+        nan = np.isnan(a)
+        r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+        r[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
+        return r
+    raise ValueError("precision must be GW_HALF or GW_BFLOAT16")
+
+
+def from_storage(a, precision):
+    """Float32 values of 16-bit storage (inverse of to_storage)."""
+    if precision == GW_HALF:
+        return np.asarray(a, dtype=np.float16).astype(np.float32)
+    if precision == GW_BFLOAT16:
+        return (np.asarray(a, dtype=np.uint16).astype(np.uint32)
+                << 16).view(np.float32)
+    raise ValueError("precision must be GW_HALF or GW_BFLOAT16")
+
+
+def grid_workload_reduced(wl, precision, comp=None):
+    """Grid a synth.Workload with a GW_HALF / GW_BFLOAT16 plan: vis stored
+    in 16 bits, float coordinates, compensated device grid.  cell_size_rad
+    and w_scale are passed rounded to float (as the FP64 oracle receives
+    them).  Returns (grid float32 [2G^2] = hi + lo, skipped, norm, the
+    compensated grid float32 [4G^2]).  ``comp`` (float32 [4G^2]) is
+    accumulated when given."""
+    g = wl.grid_size
+    plan = Plan(precision, wl.num_w_planes, wl.support, wl.oversample,
+                wl.conv_size_half, wl.conv_func,
+                float(np.float32(wl.cell_size_rad)),
+                float(np.float32(wl.w_scale)), g)
+    try:
+        d = [DeviceArray.from_host(np.ascontiguousarray(a, np.float32))
+             for a in (wl.uu, wl.vv, wl.ww)]
+        d_vis = DeviceArray.from_host(to_storage(wl.vis, precision))
+        d_comp = DeviceArray(16 * g * g)
+        if comp is None:
+            d_comp.zero()
+        else:
+            d_comp.upload(np.ascontiguousarray(comp, np.float32))
+        plan.grid(wl.num_points, *d, d_vis, d_comp)
+        sk, nd, _ = plan.results()
+        d_out = DeviceArray(8 * g * g)
+        st = ct.c_int(0)
+        lib().gw_compensated_resolve(d_comp.ptr, g * g, d_out.ptr, None,
+                                     ct.byref(st))
+        _check(st, "gw_compensated_resolve")
+        device_synchronize()
+        out = d_out.download(np.empty(2 * g * g, np.float32))
+        comp_out = d_comp.download(np.empty(4 * g * g, np.float32))
+        for x in (*d, d_vis, d_comp, d_out):
+            x.close()
+        return out, sk, nd, comp_out
+    finally:
+        plan.close()
 
 
 # --------------------------------------------------------------------------
@@ -321,7 +389,7 @@ class Plan:
     def __init__(self, precision, num_w_planes, support, oversample,
                  conv_size_half, conv_func, cell_size_rad, w_scale,
                  grid_size):
-        dt = np.float32 if precision == GW_SINGLE else np.float64
+        dt = np.float64 if precision == GW_DOUBLE else np.float32
         cube = _c(conv_func, dt)
         st = ct.c_int(0)
         self._cube = cube

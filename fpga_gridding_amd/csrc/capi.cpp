@@ -579,6 +579,20 @@ GW_EXPORT void gw_memset_device(void* ptr, int value, size_t bytes,
     });
 }
 
+GW_EXPORT void gw_compensated_resolve(const float* d_comp, size_t num_cells,
+        float* d_out, void* hip_stream, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (num_cells && (!d_comp || !d_out))
+            gw::fail(gw::kErrInvalidArgument, "device pointers must not be NULL");
+        gw::resolve_compensated(d_comp, num_cells, d_out,
+                static_cast<hipStream_t>(hip_stream));
+    });
+}
+
 GW_EXPORT void gw_device_copy(void* dst, const void* src, size_t bytes,
         int* status)
 {

@@ -109,6 +109,48 @@ __device__ __forceinline__ void atomic_add_real(double* p, double v)
     unsafeAtomicAdd(p, v);
 }
 
+// One component of a compensated two-float grid cell, p[0] = hi and
+// p[1] = lo (value hi + lo): adds x with a 64-bit compare-and-swap that
+// replaces the pair by the double sum hi + lo + x split again into
+// hi = (float)s and lo = (float)(s - hi).
+__device__ __forceinline__ void comp_add(float* p, double x)
+{
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+    unsigned long long old = __hip_atomic_load(q, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+    for (;;)
+    {
+        const float hi = __uint_as_float((unsigned)old);
+        const float lo = __uint_as_float((unsigned)(old >> 32));
+        const double sum = (double)hi + (double)lo + x;
+        const float nh = (float)sum;
+        const float nl = (float)(sum - (double)nh);
+        const unsigned long long nv = (unsigned long long)__float_as_uint(nh) |
+                ((unsigned long long)__float_as_uint(nl) << 32);
+        const unsigned long long prev = atomicCAS(q, old, nv);
+        if (prev == old) break;
+        old = prev;
+    }
+}
+
+// Adds (re, im) to cell g of the grid: interleaved (re, im) of type T, or
+// with COMP the compensated two-float layout (re_hi, re_lo, im_hi, im_lo).
+template <typename T, bool COMP>
+__device__ __forceinline__ void grid_add(T* grid, size_t g, double re,
+                                         double im)
+{
+    if constexpr (COMP)
+    {
+        comp_add(reinterpret_cast<float*>(grid) + 4 * g, re);
+        comp_add(reinterpret_cast<float*>(grid) + 4 * g + 2, im);
+    }
+    else
+    {
+        atomic_add_real(grid + 2 * g, (T)re);
+        atomic_add_real(grid + 2 * g + 1, (T)im);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {
@@ -169,15 +211,19 @@ __global__ void k_fill_u32(unsigned* __restrict__ p, size_t n)
 constexpr int kComboLds = 8192;   // per-block LDS histogram capacity
 
 // Index stage.  Follows oskar_grid_wproj.c:169-212 (FP32) and :71-101
-// (FP64) operation by operation with FMA contraction disabled.
-template <typename T, bool DIAG>
+// (FP64) operation by operation with FMA contraction disabled.  T is the
+// arithmetic of the index math; coordinates of type TU and visibilities of
+// type TV are widened to it (the reduced-precision plans: float
+// coordinates, FP16/BF16 visibilities, FP64 index math, float records).
+template <typename T, bool DIAG, typename TU = T, typename TV = T,
+          typename Rec = typename Types<T>::Rec>
 __global__ void __launch_bounds__(256) k_locate(
-        size_t n, const T* __restrict__ uu, const T* __restrict__ vv,
-        const T* __restrict__ ww, const T* __restrict__ vis, T grid_scale,
+        size_t n, const TU* __restrict__ uu, const TU* __restrict__ vv,
+        const TU* __restrict__ ww, const TV* __restrict__ vis, T grid_scale,
         T w_scale, int ov, int G, int P, const int* __restrict__ support,
         int h, int K, int tile_bits, int combo_bits, int key_bits,
         unsigned long long* __restrict__ keys, unsigned* __restrict__ idx,
-        typename Types<T>::Rec* __restrict__ rec,
+        Rec* __restrict__ rec,
         unsigned* __restrict__ combo, Summary* __restrict__ summary,
         unsigned* __restrict__ gw_out,
         unsigned long long* __restrict__ diag_support,
@@ -208,9 +254,9 @@ __global__ void __launch_bounds__(256) k_locate(
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += stride)
     {
-        const T pu = -uu[i] * grid_scale;
-        const T pv = vv[i] * grid_scale;
-        const T w = ww[i];
+        const T pu = -(T)uu[i] * grid_scale;
+        const T pv = (T)vv[i] * grid_scale;
+        const T w = (T)ww[i];
         const T ru = d_round(pu);
         const T rv = d_round(pv);
         const unsigned long long grid_w =
@@ -258,13 +304,13 @@ __global__ void __launch_bounds__(256) k_locate(
             vmax = max(vmax, v0 + 2 * S);
             umin = min(umin, u0);
             umax = max(umax, u0 + 2 * S);
-            typename Types<T>::Rec r;
+            Rec r;
             r.pos = (int)((unsigned)u0 | ((unsigned)v0 << 16));
             r.sub = (int)((unsigned)(off_u + 128) |
                     ((unsigned)(off_v + 128) << 8) | ((unsigned)conj << 16) |
                     ((unsigned)plane << 17));
-            r.re = vis[2 * i];
-            r.im = vis[2 * i + 1];
+            r.re = to_real(vis[2 * i]);
+            r.im = to_real(vis[2 * i + 1]);
             rec[i] = r;
         }
         keys[i] = key;
@@ -618,11 +664,10 @@ static_assert(kImBytes < 65536, "imaginary offset must fit a DS offset");
 template <typename T>
 struct ScatterCtx
 {
-    typedef typename Types<T>::T2 T2;
     typedef typename Types<T>::T4 T4;
     const int4* __restrict__ meta;
     const T4* __restrict__ vq;
-    const T2* __restrict__ etab;
+    const void* __restrict__ etab;   // tap pairs of type TapT
     GridLds* lds;
     T* __restrict__ grid;
     unsigned* __restrict__ diag_count;
@@ -668,18 +713,17 @@ __device__ __forceinline__ void settle(T& v)
 // overlap work.  In the LDS path lanes past the footprint are not masked
 // off: their sums go to a per-lane sink cell after the region (kSinkCells
 // of them), which is never read.
-template <typename T, bool DIAG, bool LDS, int NS>
+template <typename T, bool DIAG, bool LDS, int NS, typename TapT, bool COMP>
 __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
         const PlaneInfo& pi, int wis, int a, int b, int cb)
 {
-    typedef typename Types<T>::T2 T2;
     typedef typename Types<T>::T4 T4;
     const int lane = threadIdx.x & 63;
     const int W = pi.W, W2 = W * W;
     const int stride = 64 * pi.wpp;
-    const char* __restrict__ ptab =
-            reinterpret_cast<const char*>(x.etab + pi.eoff);
-    const size_t table_bytes = (size_t)W2 * sizeof(T2);
+    const char* __restrict__ ptab = reinterpret_cast<const char*>(
+            static_cast<const TapT*>(x.etab) + pi.eoff);
+    const size_t table_bytes = (size_t)W2 * sizeof(TapT);
     const int cstride = LDS ? x.R : x.G;
     unsigned toff[NS];     // byte offset of the lane's tap in a table
     unsigned loff[NS];     // LDS: byte offset from the footprint origin in
@@ -701,7 +745,7 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
             const unsigned j = __umulhi(cc, pi.wmagic);
             const unsigned k = cc - j * (unsigned)W;
             loff[s] = j * (unsigned)cstride + k;
-            toff[s] = cc * (unsigned)sizeof(T2);
+            toff[s] = cc * (unsigned)sizeof(TapT);
         }
     }
     else
@@ -715,7 +759,7 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
             const unsigned j = cc / (unsigned)W;
             const unsigned k = cc - j * (unsigned)W;
             loff[s] = j * (unsigned)cstride + k;
-            toff[s] = cc * (unsigned)sizeof(T2);
+            toff[s] = cc * (unsigned)sizeof(TapT);
         }
     }
     auto load_taps = [&](int table) {
@@ -723,9 +767,10 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
 #pragma unroll
         for (int s = 0; s < NS; ++s)
         {
-            const T2 t = *reinterpret_cast<const T2*>(tp + toff[s]);
-            nre[s] = t.x;
-            nim[s] = t.y;
+            const auto t = tap_pair(
+                    *reinterpret_cast<const TapT*>(tp + toff[s]));
+            nre[s] = (T)t.x;
+            nim[s] = (T)t.y;
         }
     };
     auto take_taps = [&]() {
@@ -768,8 +813,8 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
             else if (valid[s])
             {
                 const size_t g = (size_t)base + loff[s];
-                atomic_add_real(x.grid + 2 * g, (T)acc_re[s]);
-                atomic_add_real(x.grid + 2 * g + 1, (T)acc_im[s]);
+                grid_add<T, COMP>(x.grid, g, (double)acc_re[s],
+                                  (double)acc_im[s]);
                 if (DIAG)
                 {
                     atomicAdd(&x.diag_count[g], cnt);
@@ -854,33 +899,112 @@ template <typename T> constexpr int slots_upto(int n)
     return n < slots_max<T>() ? n : slots_max<T>();
 }
 
-template <typename T, bool DIAG, bool LDS>
+template <typename T, bool DIAG, bool LDS, typename TapT, bool COMP>
 __device__ __forceinline__ void scatter_dispatch(const ScatterCtx<T>& x,
         const PlaneInfo& pi, int wis, int a, int b, int cb)
 {
     // The plan never sets nslot above slots_max<T>() (<= 8).
     static_assert(slots_max<T>() <= 8, "dispatch covers 1..8 slots");
+#define GW_STREAM(n) scatter_stream<T, DIAG, LDS, slots_upto<T>(n), TapT, \
+        COMP>(x, pi, wis, a, b, cb)
     switch (pi.nslot)
     {
-    case 1: scatter_stream<T, DIAG, LDS, 1>(x, pi, wis, a, b, cb); break;
-    case 2: scatter_stream<T, DIAG, LDS, slots_upto<T>(2)>(x, pi, wis, a, b, cb); break;
-    case 3: scatter_stream<T, DIAG, LDS, slots_upto<T>(3)>(x, pi, wis, a, b, cb); break;
-    case 4: scatter_stream<T, DIAG, LDS, slots_upto<T>(4)>(x, pi, wis, a, b, cb); break;
-    case 5: scatter_stream<T, DIAG, LDS, slots_upto<T>(5)>(x, pi, wis, a, b, cb); break;
-    case 6: scatter_stream<T, DIAG, LDS, slots_upto<T>(6)>(x, pi, wis, a, b, cb); break;
-    case 7: scatter_stream<T, DIAG, LDS, slots_upto<T>(7)>(x, pi, wis, a, b, cb); break;
-    default: scatter_stream<T, DIAG, LDS, slots_upto<T>(8)>(x, pi, wis, a, b, cb); break;
+    case 1: GW_STREAM(1); break;
+    case 2: GW_STREAM(2); break;
+    case 3: GW_STREAM(3); break;
+    case 4: GW_STREAM(4); break;
+    case 5: GW_STREAM(5); break;
+    case 6: GW_STREAM(6); break;
+    case 7: GW_STREAM(7); break;
+    default: GW_STREAM(8); break;
+    }
+#undef GW_STREAM
+}
+
+// Region -> compensated grid (re_hi, re_lo, im_hi, im_lo per cell), kBatch
+// cells per thread at a time: the current pairs of all of them are read,
+// then one compare-and-swap each is issued, so the round trips overlap
+// (a pair changed meanwhile by another workgroup is retried alone).
+// Clears the region.  y0, x0: grid row (relative to the rows held) and
+// column of region cell 0.
+__device__ __forceinline__ void flush_region_comp(GridLds& L, int R, int y0,
+        int x0, int G, float* grid)
+{
+    constexpr int kBatch = 4;
+    const int NT = blockDim.x;
+    const int cells = R * R;
+    for (int base = threadIdx.x; base < cells; base += NT * kBatch)
+    {
+        unsigned long long* q[2 * kBatch];
+        unsigned long long old[2 * kBatch];
+        double add[2 * kBatch];
+        bool live[2 * kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b)
+        {
+            const int i = base + b * NT;
+            double re = 0.0, im = 0.0;
+            if (i < cells)
+            {
+                re = L.re[i];
+                im = L.im[i];
+            }
+            const bool nz = re != 0.0 || im != 0.0;
+            live[2 * b] = live[2 * b + 1] = nz;
+            add[2 * b] = re;
+            add[2 * b + 1] = im;
+            q[2 * b] = q[2 * b + 1] = nullptr;
+            if (nz)
+            {
+                L.re[i] = 0.0;
+                L.im[i] = 0.0;
+                const int y = i / R;
+                const int xx = i - y * R;
+                const size_t g = (size_t)((long long)(y0 + y) * G + (x0 + xx));
+                q[2 * b] = reinterpret_cast<unsigned long long*>(grid + 4 * g);
+                q[2 * b + 1] = q[2 * b] + 1;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 2 * kBatch; ++c)
+            old[c] = live[c] ? __hip_atomic_load(q[c], __ATOMIC_RELAXED,
+                    __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        auto pack = [](unsigned long long o, double x) {
+            const float hi = __uint_as_float((unsigned)o);
+            const float lo = __uint_as_float((unsigned)(o >> 32));
+            const double sum = (double)hi + (double)lo + x;
+            const float nh = (float)sum;
+            const float nl = (float)(sum - (double)nh);
+            return (unsigned long long)__float_as_uint(nh) |
+                    ((unsigned long long)__float_as_uint(nl) << 32);
+        };
+        unsigned long long prev[2 * kBatch];
+#pragma unroll
+        for (int c = 0; c < 2 * kBatch; ++c)
+            prev[c] = live[c] ? atomicCAS(q[c], old[c], pack(old[c], add[c]))
+                              : old[c];
+#pragma unroll
+        for (int c = 0; c < 2 * kBatch; ++c)
+        {
+            while (prev[c] != old[c])
+            {
+                old[c] = prev[c];
+                prev[c] = atomicCAS(q[c], old[c], pack(old[c], add[c]));
+            }
+        }
     }
 }
 
 // Grid-stride loop over the jobs (persistent: the plan launches as many
-// workgroups as fit the device at once).
-template <typename T, bool DIAG>
+// workgroups as fit the device at once).  Taps are stored as TapT pairs;
+// COMP selects the compensated two-float grid layout.
+template <typename T, bool DIAG, typename TapT = typename Types<T>::T2,
+          bool COMP = false>
 __global__ void __launch_bounds__(kGridThreads) k_grid(
         const Summary* __restrict__ summary, const int4* __restrict__ items,
         const PlaneInfo* __restrict__ pinfo, const int4* __restrict__ meta,
         const typename Types<T>::T4* __restrict__ vq,
-        const typename Types<T>::T2* __restrict__ etab, int G, int row0,
+        const TapT* __restrict__ etab, int G, int row0,
         int R, T* __restrict__ grid, int force_direct, int min_share,
         unsigned* __restrict__ diag_count,
         unsigned long long* __restrict__ diag_wsum)
@@ -973,9 +1097,11 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
             for (int cb = 0; cb < pi.W * pi.W; cb += pass)
             {
                 if (lds && !DIAG)
-                    scatter_dispatch<T, DIAG, true>(x, pi, wis, sa, sb, cb);
+                    scatter_dispatch<T, DIAG, true, TapT, COMP>(x, pi, wis, sa,
+                            sb, cb);
                 else
-                    scatter_dispatch<T, DIAG, false>(x, pi, wis, sa, sb, cb);
+                    scatter_dispatch<T, DIAG, false, TapT, COMP>(x, pi, wis, sa,
+                            sb, cb);
             }
         }
         __syncthreads();
@@ -984,22 +1110,25 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
             // Region -> grid, coalesced along rows, clearing it for the next
             // job.  Cells no footprint reached hold zero and are skipped
             // (they may lie off the grid).
-            for (int i = tid; i < R * R; i += NT)
-            {
-                const double re = L.re[i];
-                const double im = L.im[i];
-                if (re != 0.0 || im != 0.0)
+            if constexpr (COMP)
+                flush_region_comp(L, R, x.vy0 - row0, x.ux0, G,
+                                  reinterpret_cast<float*>(grid));
+            else
+                for (int i = tid; i < R * R; i += NT)
                 {
-                    L.re[i] = 0.0;
-                    L.im[i] = 0.0;
-                    const int y = i / R;
-                    const int xx = i - y * R;
-                    const size_t g = (size_t)(x.vy0 - row0 + y) * (size_t)G +
-                            (size_t)(x.ux0 + xx);
-                    atomic_add_real(grid + 2 * g, (T)re);
-                    atomic_add_real(grid + 2 * g + 1, (T)im);
+                    const double re = L.re[i];
+                    const double im = L.im[i];
+                    if (re != 0.0 || im != 0.0)
+                    {
+                        L.re[i] = 0.0;
+                        L.im[i] = 0.0;
+                        const int y = i / R;
+                        const int xx = i - y * R;
+                        const size_t g = (size_t)(x.vy0 - row0 + y) *
+                                (size_t)G + (size_t)(x.ux0 + xx);
+                        grid_add<T, COMP>(grid, g, re, im);
+                    }
                 }
-            }
             __syncthreads();
         }
     }
@@ -1043,6 +1172,31 @@ void device_copy(void* dst, const void* src, size_t bytes)
             static_cast<const unsigned char*>(src) + n16 * 16, tail);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_copy launch");
     check_hip(hipDeviceSynchronize(), kErrKernelLaunch, "k_copy");
+}
+
+// Compensated two-float grid -> float grid: out[2q + c] = hi + lo of
+// component c of cell q (layout re_hi, re_lo, im_hi, im_lo per cell).
+__global__ void __launch_bounds__(256) k_resolve(const float2* __restrict__ comp,
+        float* __restrict__ out, size_t n_components)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+         i < n_components; i += stride)
+    {
+        const float2 p = comp[i];
+        out[i] = p.x + p.y;
+    }
+}
+
+void resolve_compensated(const float* comp, size_t num_cells, float* out,
+                         hipStream_t stream)
+{
+    if (num_cells == 0) return;
+    const size_t n = 2 * num_cells;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, stream,
+            reinterpret_cast<const float2*>(comp), out, n);
+    check_hip(hipGetLastError(), kErrKernelLaunch, "k_resolve launch");
 }
 
 // -------------------------------------------------------------------------
@@ -1166,8 +1320,9 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
            int oversample, int conv_size_half, const void* conv_func,
            double cell_size_rad, double w_scale, int grid_size)
 {
-    if (precision != 0x01 && precision != 0x02)
-        fail(kErrInvalidArgument, "precision must be GW_SINGLE or GW_DOUBLE");
+    if (precision < kPrecSingle || precision > kPrecBFloat16)
+        fail(kErrInvalidArgument, "precision must be GW_SINGLE, GW_DOUBLE, "
+             "GW_HALF or GW_BFLOAT16");
     validate_args(num_w_planes, support, oversample, conv_size_half,
             conv_func, grid_size);
     prec_ = precision;
@@ -1211,8 +1366,8 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
 
     // Streams per plane: wpp waves cover the footprint with <= smax cells
     // per lane (larger footprints take several passes).
-    const int smax = precision == 0x01 ? slots_max<float>() :
-            slots_max<double>();
+    const int smax = precision == kPrecDouble ? slots_max<double>() :
+            slots_max<float>();
     pinfo_.resize(P_);
     for (size_t p = 0; p < P_; ++p)
     {
@@ -1253,8 +1408,8 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
             kErrComputeDevices, "hipDeviceGetAttribute");
     int occ = 0;
     check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ,
-            precision == 0x01 ? (const void*)k_grid<float, false> :
-            (const void*)k_grid<double, false>, kGridThreads, 0),
+            precision == kPrecDouble ? (const void*)k_grid<double, false> :
+            (const void*)k_grid<float, false>, kGridThreads, 0),
             kErrComputeDevices, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     grid_blocks_ = n_cu_ * std::max(1, occ);
     check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
@@ -1269,13 +1424,51 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     std::vector<double> etab_d;
     const void* etab_host = nullptr;
     size_t etab_bytes = 0;
-    if (prec_ == 0x01)
+    std::vector<HalfPair> etab_h;
+    std::vector<BF16Pair> etab_b;
+    if (prec_ == kPrecSingle)
     {
         build_tables(static_cast<const float*>(conv_func), P_,
                 support_.data(), ov_, C_, h_, pinfo_, etab_f, normtab);
         etab_host = etab_f.data();
         etab_bytes = etab_f.size() * sizeof(float);
         etab_taps_ = etab_f.size() / 2;
+    }
+    else if (prec_ == kPrecHalf || prec_ == kPrecBFloat16)
+    {
+        // Taps are stored in 16 bits: round the cube first, so that the
+        // tables and the norm table hold exactly the stored values.
+        const float* cf = static_cast<const float*>(conv_func);
+        std::vector<float> cube(cf, cf + 2 * P_ * (size_t)C_ * C_);
+        for (float& v : cube)
+            v = prec_ == kPrecHalf ? (float)(half_t)v :
+                    bf16_to_float(float_to_bf16(v));
+        build_tables(cube.data(), P_, support_.data(), ov_, C_, h_, pinfo_,
+                etab_f, normtab);
+        etab_taps_ = etab_f.size() / 2;
+        if (prec_ == kPrecHalf)
+        {
+            etab_h.resize(etab_taps_);
+            for (size_t i = 0; i < etab_taps_; ++i)
+            {
+                etab_h[i].x = (half_t)etab_f[2 * i];
+                etab_h[i].y = (half_t)etab_f[2 * i + 1];
+            }
+            etab_host = etab_h.data();
+            etab_bytes = etab_h.size() * sizeof(HalfPair);
+        }
+        else
+        {
+            etab_b.resize(etab_taps_);
+            for (size_t i = 0; i < etab_taps_; ++i)
+            {
+                etab_b[i].x = float_to_bf16(etab_f[2 * i]);
+                etab_b[i].y = float_to_bf16(etab_f[2 * i + 1]);
+            }
+            etab_host = etab_b.data();
+            etab_bytes = etab_b.size() * sizeof(BF16Pair);
+        }
+        std::vector<float>().swap(etab_f);
     }
     else
     {
@@ -1372,19 +1565,22 @@ void Plan::free_workspace()
 
 void Plan::ensure_capacity(size_t n)
 {
-    if (n <= cap_) return;
+    // The planning kernels write the scan partials and the bucket table
+    // even for zero points, so the workspace exists from the first call.
+    if (cap_ > 0 && n <= cap_) return;
     if (n >= 0x7FFFFFFFull)
         fail(kErrInvalidArgument, "num_points must be below 2^31 per call");
     free_workspace();
     const size_t cap = std::max<size_t>(n, 1024);
-    const size_t real_size = prec_ == 0x01 ? sizeof(float) : sizeof(double);
+    const size_t real_size = prec_ == kPrecDouble ? sizeof(double) :
+            sizeof(float);
     for (int i = 0; i < 2; ++i)
     {
         d_keys_[i] = dmalloc<unsigned long long>(cap);
         d_idx_[i] = dmalloc<unsigned>(cap);
     }
     d_rec_ = dmalloc<char>(cap *
-            (prec_ == 0x01 ? sizeof(RecF) : sizeof(RecD)));
+            (prec_ == kPrecDouble ? sizeof(RecD) : sizeof(RecF)));
     d_meta_ = dmalloc<int4>(cap);
     d_vq_ = dmalloc<char>(cap * 4 * real_size);
     d_gw_ = dmalloc<unsigned>(cap);
@@ -1418,9 +1614,9 @@ void Plan::scan_u32(const unsigned* in, unsigned* out, const unsigned* count,
     check_hip(hipGetLastError(), kErrKernelLaunch, "scan launch");
 }
 
-template <typename T>
-void Plan::locate_t(size_t n, const T* uu, const T* vv, const T* ww,
-                    const T* vis, const DiagBuffers* diag)
+template <typename T, typename TU, typename TV, typename Rec>
+void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
+                    const TV* vis, const DiagBuffers* diag)
 {
     ensure_capacity(n);
     timed_ = true;
@@ -1433,16 +1629,15 @@ void Plan::locate_t(size_t n, const T* uu, const T* vv, const T* ww,
             (n + 255) / 256, (size_t)n_cu_ * 4));
     const T gscale = sizeof(T) == 4 ? (T)gscale_f_ : (T)gscale_d_;
     const T wsc = sizeof(T) == 4 ? (T)ws_f_ : (T)ws_d_;
-    typedef typename Types<T>::Rec Rec;
     if (diag)
-        hipLaunchKernelGGL((k_locate<T, true>), dim3(blocks), dim3(256), 0,
+        hipLaunchKernelGGL((k_locate<T, true, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
                 d_support_, h_, K_, tile_bits_, combo_bits_, key_bits_,
                 d_keys_[0], d_idx_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, d_gw_, diag->per_support,
                 diag->per_plane, diag->plane_cap);
     else
-        hipLaunchKernelGGL((k_locate<T, false>), dim3(blocks), dim3(256), 0,
+        hipLaunchKernelGGL((k_locate<T, false, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
                 d_support_, h_, K_, tile_bits_, combo_bits_, key_bits_,
                 d_keys_[0], d_idx_[0],
@@ -1453,7 +1648,7 @@ void Plan::locate_t(size_t n, const T* uu, const T* vv, const T* ww,
     check_hip(hipEventRecord(ev_[1], stream_), kErrKernelLaunch, "event");
 }
 
-template <typename T>
+template <typename T, typename TapT, bool COMP>
 void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
 {
     typedef typename Types<T>::Rec Rec;
@@ -1505,11 +1700,11 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
                 (const T4*)d_vq_, (const T2*)d_etab_, G_, row0, 0, grid, 1,
                 min_share_, diag->per_cell, diag->wsum_cell);
     else
-        hipLaunchKernelGGL((k_grid<T, false>), dim3(grid_blocks_),
+        hipLaunchKernelGGL((k_grid<T, false, TapT, COMP>), dim3(grid_blocks_),
                 dim3(kGridThreads), 0, stream_,
                 (const Summary*)d_summary_, (const int4*)d_items_,
                 (const PlaneInfo*)d_pinfo_, (const int4*)d_meta_,
-                (const T4*)d_vq_, (const T2*)d_etab_, G_, row0, region_, grid,
+                (const T4*)d_vq_, (const TapT*)d_etab_, G_, row0, region_, grid,
                 0, min_share_, (unsigned*)nullptr,
                 (unsigned long long*)nullptr);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_grid launch");
@@ -1519,20 +1714,37 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
 void Plan::locate(size_t n, const void* uu, const void* vv, const void* ww,
                   const void* vis, const DiagBuffers* diag)
 {
-    if (prec_ == 0x01)
+    if (prec_ == kPrecSingle)
         locate_t<float>(n, (const float*)uu, (const float*)vv,
                 (const float*)ww, (const float*)vis, diag);
-    else
+    else if (prec_ == kPrecDouble)
         locate_t<double>(n, (const double*)uu, (const double*)vv,
                 (const double*)ww, (const double*)vis, diag);
+    else
+    {
+        if (diag)
+            fail(kErrInvalidArgument, "diagnostics need a GW_SINGLE plan");
+        if (prec_ == kPrecHalf)
+            locate_t<double, float, half_t, RecF>(n, (const float*)uu,
+                    (const float*)vv, (const float*)ww, (const half_t*)vis,
+                    nullptr);
+        else
+            locate_t<double, float, bf16_t, RecF>(n, (const float*)uu,
+                    (const float*)vv, (const float*)ww, (const bf16_t*)vis,
+                    nullptr);
+    }
 }
 
 void Plan::scatter(size_t n, void* grid, int row0, const DiagBuffers* diag)
 {
-    if (prec_ == 0x01)
+    if (prec_ == kPrecSingle)
         scatter_t<float>(n, (float*)grid, row0, diag);
-    else
+    else if (prec_ == kPrecDouble)
         scatter_t<double>(n, (double*)grid, row0, diag);
+    else if (prec_ == kPrecHalf)
+        scatter_t<float, HalfPair, true>(n, (float*)grid, row0, nullptr);
+    else
+        scatter_t<float, BF16Pair, true>(n, (float*)grid, row0, nullptr);
 }
 
 const Summary& Plan::read_summary()

@@ -73,6 +73,63 @@ struct Summary
 struct alignas(16) RecF { int pos; int sub; float re; float im; };
 struct alignas(16) RecD { int pos; int sub; int pad0, pad1; double re, im; };
 
+// Plan precisions (values of gw_precision in include/fpga_gridding_amd.h).
+enum
+{
+    kPrecSingle = 0x01,     // FP32 throughout (oskar_grid_wproj_f)
+    kPrecDouble = 0x02,     // FP64 throughout (oskar_grid_wproj_d)
+    kPrecHalf = 0x03,       // FP16 taps and visibilities, FP64 index math,
+                            // FP32 products, compensated two-float grid
+    kPrecBFloat16 = 0x04    // as kPrecHalf with BF16 storage
+};
+
+// 16-bit storage types.  FP16 is the compiler's _Float16; BF16 is the top
+// half of an FP32 (conversion to float is a shift, from float rounds to
+// nearest even).
+typedef _Float16 half_t;
+struct bf16_t { unsigned short bits; };
+struct alignas(4) HalfPair { half_t x, y; };
+struct alignas(4) BF16Pair { unsigned short x, y; };
+
+__host__ __device__ inline float bf16_to_float(unsigned short b)
+{
+    const unsigned u = (unsigned)b << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+inline unsigned short float_to_bf16(float f)
+{
+    unsigned u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu))
+        return (unsigned short)((u >> 16) | 0x0040u);      // quiet NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+// Real value of an input element.
+__host__ __device__ inline float to_real(float x) { return x; }
+__host__ __device__ inline double to_real(double x) { return x; }
+__host__ __device__ inline float to_real(half_t x) { return (float)x; }
+__host__ __device__ inline float to_real(bf16_t x)
+{
+    return bf16_to_float(x.bits);
+}
+
+// A stored tap pair as float2 / double2.
+__host__ __device__ inline float2 tap_pair(const float2& t) { return t; }
+__host__ __device__ inline double2 tap_pair(const double2& t) { return t; }
+__host__ __device__ inline float2 tap_pair(const HalfPair& t)
+{
+    return make_float2((float)t.x, (float)t.y);
+}
+__host__ __device__ inline float2 tap_pair(const BF16Pair& t)
+{
+    return make_float2(bf16_to_float(t.x), bf16_to_float(t.y));
+}
+
 template <typename T> struct Types;
 template <> struct Types<float>
 {
@@ -108,6 +165,11 @@ struct Error
 void check_hip(hipError_t e, int code, const char* what);
 
 // Host-side argument checks of a gridding call (no device access): raises
+// Compensated two-float grid (re_hi, re_lo, im_hi, im_lo per cell) to a
+// float grid (re, im per cell), enqueued on `stream`.
+void resolve_compensated(const float* comp, size_t num_cells, float* out,
+                         hipStream_t stream);
+
 // Streaming device-to-device copy (k_copy) on the null stream; waits.
 void device_copy(void* dst, const void* src, size_t bytes);
 
@@ -155,10 +217,13 @@ public:
 private:
     void ensure_capacity(size_t n);
     void free_workspace();
-    template <typename T> void locate_t(size_t n, const T* uu, const T* vv,
-            const T* ww, const T* vis, const DiagBuffers* diag);
-    template <typename T> void scatter_t(size_t n, T* grid, int row0,
-            const DiagBuffers* diag);
+    template <typename T, typename TU = T, typename TV = T,
+              typename Rec = typename Types<T>::Rec>
+    void locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
+                  const TV* vis, const DiagBuffers* diag);
+    template <typename T, typename TapT = typename Types<T>::T2,
+              bool COMP = false>
+    void scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag);
     void scan_u32(const unsigned* in, unsigned* out, const unsigned* count,
                   size_t max_n, unsigned* total);
 

@@ -41,8 +41,15 @@ enum gw_status_code
     GW_ERR_COMPUTE_DEVICES        = -19   /* OSKAR_ERR_COMPUTE_DEVICES */
 };
 
-/* Precision of a plan (values of OSKAR_SINGLE / OSKAR_DOUBLE). */
-enum gw_precision { GW_SINGLE = 0x01, GW_DOUBLE = 0x02 };
+/* Precision of a plan.  GW_SINGLE and GW_DOUBLE have the values of
+ * OSKAR_SINGLE / OSKAR_DOUBLE and the reference's arithmetic
+ * (oskar_grid_wproj_f / _d).  GW_HALF and GW_BFLOAT16 are the reduced-
+ * precision forms, which have no counterpart in the reference: taps and
+ * visibilities stored in FP16 / BF16, FP64 index math (the indices of
+ * oskar_grid_wproj_d), FP32 products, FP64 per-job sums and a compensated
+ * two-float grid; the norm is summed in FP64 over the stored taps. */
+enum gw_precision { GW_SINGLE = 0x01, GW_DOUBLE = 0x02, GW_HALF = 0x03,
+                    GW_BFLOAT16 = 0x04 };
 
 /* ---------------------------------------------------------------------- */
 /* Status-returning drop-in calls                                          */
@@ -89,10 +96,11 @@ typedef struct gw_plan gw_plan;
 
 /* Create a plan on the current HIP device.  Arguments have the meaning of
  * the reference gridder's (oskar_grid_wproj.h:50-63); conv_func is a HOST
- * array of 2*num_w_planes*conv_size_half^2 values of the plan's precision
- * (float for GW_SINGLE, double for GW_DOUBLE) in the reference layout
- * conv_func[2*((p*C + iy)*C + ix) + {0,1}].  For GW_SINGLE, cell_size_rad
- * and w_scale are rounded to float as the FP32 gridder receives them. */
+ * array of 2*num_w_planes*conv_size_half^2 values (double for GW_DOUBLE,
+ * float otherwise; GW_HALF / GW_BFLOAT16 round them to 16 bits) in the
+ * reference layout conv_func[2*((p*C + iy)*C + ix) + {0,1}].  For
+ * GW_SINGLE, cell_size_rad and w_scale are rounded to float as the FP32
+ * gridder receives them; the other precisions use them as given. */
 GW_EXPORT
 gw_plan* gw_plan_create(int precision, size_t num_w_planes,
         const int* support, int oversample, int conv_size_half,
@@ -105,11 +113,17 @@ GW_EXPORT void gw_plan_free(gw_plan* plan);
  * non-blocking stream).  The caller keeps ownership of the stream. */
 GW_EXPORT void gw_plan_set_stream(gw_plan* plan, void* hip_stream);
 
-/* Grid num_points points held in DEVICE memory into a DEVICE grid of
- * 2*grid_size^2 values of the plan's precision (accumulated, row-major
- * complex).  d_vis holds 2*num_points values; weights are not used, as in
- * the reference.  Asynchronous on the plan's stream: read num_skipped and
- * the norm increment with gw_plan_results(). */
+/* Grid num_points points held in DEVICE memory into a DEVICE grid
+ * (accumulated, row-major complex).  Weights are not used, as in the
+ * reference.  Asynchronous on the plan's stream: read num_skipped and the
+ * norm increment with gw_plan_results().  Element types by precision:
+ *   GW_SINGLE   uu/vv/ww float, d_vis 2*num_points float,
+ *               d_grid 2*grid_size^2 float (re, im);
+ *   GW_DOUBLE   the same with double;
+ *   GW_HALF     uu/vv/ww float, d_vis 2*num_points FP16 (_Float16),
+ *               d_grid 4*grid_size^2 float: per cell re_hi, re_lo, im_hi,
+ *               im_lo, the value being hi + lo (gw_compensated_resolve);
+ *   GW_BFLOAT16 as GW_HALF with d_vis in BF16 (top 16 bits of an FP32). */
 GW_EXPORT
 void gw_grid_device(gw_plan* plan, size_t num_points, const void* d_uu,
         const void* d_vv, const void* d_ww, const void* d_vis, void* d_grid,
@@ -146,6 +160,12 @@ GW_EXPORT void  gw_memcpy_d2h(void* dst, const void* src, size_t bytes,
 GW_EXPORT void  gw_memset_device(void* ptr, int value, size_t bytes,
         int* status);
 GW_EXPORT void  gw_device_synchronize(int* status);
+/* Compensated grid (4 floats per cell, re_hi, re_lo, im_hi, im_lo) to a
+ * float grid (re, im per cell): d_out[2q + c] = hi + lo.  Enqueued on the
+ * given hipStream_t (NULL: the null stream). */
+GW_EXPORT void  gw_compensated_resolve(const float* d_comp, size_t num_cells,
+        float* d_out, void* hip_stream, int* status);
+
 /* Device-to-device copy with the library's own streaming kernel (k_copy,
  * 16-byte loads and stores); synchronous.  Besides plain use it calibrates
  * the HBM byte counters of a profiler: a copy of B bytes moves exactly B

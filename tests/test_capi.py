@@ -125,3 +125,25 @@ def test_no_device_reports_compute_device_error():
     with pytest.raises(fg.GridderError) as e:
         fg.grid_workload_f(_wl())
     assert e.value.status == -19
+
+
+@pytest.mark.parametrize("precision", [0, 5, -1])
+def test_plan_rejects_unknown_precision(precision):
+    """gw_plan_create checks the precision code before any GPU call."""
+    _ensure_built()
+    wl = _wl()
+    with pytest.raises(fg.GridderError) as e:
+        fg.Plan.for_workload(wl, precision=precision)
+    assert e.value.status == -3
+    assert "precision" in fg.lib().gw_last_error_detail().decode()
+
+
+def test_storage_rounding_is_nearest_even():
+    """The 16-bit storage conversions the reduced-precision plans use."""
+    a = np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -2.5, 65504.0,
+                  1e-8], np.float32)
+    bf = fg.from_storage(fg.to_storage(a, fg.GW_BFLOAT16), fg.GW_BFLOAT16)
+    assert list(bf[:4]) == [1.0, 1.0, 1.0 + 2.0 ** -6, -2.5]
+    h = fg.from_storage(fg.to_storage(a, fg.GW_HALF), fg.GW_HALF)
+    assert list(h[:5]) == [1.0, 1.00390625, 1.01171875, -2.5, 65504.0]
+    assert h[5] == 0.0

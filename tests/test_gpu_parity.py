@@ -216,6 +216,26 @@ def test_resident_plan_matches_dropin():
     assert nu == U
 
 
+def test_resident_plan_empty_first_call():
+    """A fresh plan's first call with zero points grids nothing and leaves
+    the plan usable (its workspace exists from the first call)."""
+    wl = synth.random_small(n=500, grid_size=64, num_planes=4, s_min=1,
+                            s_max=4, seed=21)
+    pl = fg.Plan.for_workload(wl)
+    dg = fg.DeviceArray(8 * wl.grid_size ** 2)
+    dg.zero()
+    pl.grid(0, None, None, None, None, dg)
+    assert pl.results() == (0, 0.0, 0)
+    d = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww, wl.vis)]
+    pl.grid(wl.num_points, *d, dg)
+    sk, nd, _ = pl.results()
+    g = dg.download(np.empty(2 * wl.grid_size ** 2, np.float32))
+    g_ref, sk_ref, n_ref = fg.grid_workload_f(wl)
+    assert sk == sk_ref and abs(nd - n_ref) <= 1e-12 * abs(n_ref)
+    assert max_rel(g, g_ref) <= 2e-6
+    pl.close()
+
+
 # --------------------------------------------------------------------------
 # Full scale: 512 stations, 13,081,600 points, 8192^2, 128 planes
 # --------------------------------------------------------------------------
