@@ -70,13 +70,20 @@ def accuracy_sample(wl, n_sample=200_000, seed=4):
                               vis=np.ascontiguousarray(vis))
     g, _, _ = fg.grid_workload_f(sub)
     out = {"sample": f"{n_sample} random points of the workload"}
-    for name, fn in (("fp64_oracle", oracle.grid_wproj_d),
-                     ("exact_oracle", oracle.grid_wproj_f_exact)):
-        ref = fn(sub)[0].astype(np.float64)
-        d = g.astype(np.float64) - ref
-        out[name] = {"max_rel": float(np.abs(d).max() / np.abs(ref).max()),
-                     "rms_rel": float(np.sqrt(np.mean(d * d)) /
-                                      np.sqrt(np.mean(ref * ref)))}
+
+    def err(a, ref):
+        d = a.astype(np.float64) - ref
+        return {"max_rel": float(np.abs(d).max() / np.abs(ref).max()),
+                "rms_rel": float(np.sqrt(np.mean(d * d)) /
+                                 np.sqrt(np.mean(ref * ref)))}
+
+    ref_d = oracle.grid_wproj_d(sub)[0]
+    out["fp64_oracle"] = err(g, ref_d)
+    out["exact_oracle"] = err(g, oracle.grid_wproj_f_exact(sub)[0])
+    # Reduced-precision plans (FP64 index math, so the FP64 oracle's cells).
+    for prec, name in ((fg.GW_HALF, "fp16"), (fg.GW_BFLOAT16, "bf16")):
+        out[f"{name}_vs_fp64_oracle"] = err(
+            fg.grid_workload_reduced(sub, prec)[0], ref_d)
     return out
 
 
@@ -141,6 +148,43 @@ def cpu_baseline(wl, budget_s=12.0, seed=3):
                 updates_per_s=done_upd / elapsed, seconds=elapsed)
 
 
+def precision_variants(wl, d_uvw, args):
+    """Mvis/s of the FP16 and BF16 plans on the same resident points (16-bit
+    visibilities, compensated two-float grid), same steps as the headline."""
+    import fpga_gridding_amd as fg
+    out = {}
+    G = wl.grid_size
+    for prec, name in ((fg.GW_HALF, "fp16"), (fg.GW_BFLOAT16, "bf16")):
+        plan = fg.Plan(prec, wl.num_w_planes, wl.support, wl.oversample,
+                       wl.conv_size_half, wl.conv_func,
+                       float(np.float32(wl.cell_size_rad)),
+                       float(np.float32(wl.w_scale)), G)
+        d_vis = fg.DeviceArray.from_host(fg.to_storage(wl.vis, prec))
+        d_comp = fg.DeviceArray(16 * G * G)
+        d_comp.zero()
+        for _ in range(args.warmup):
+            plan.grid(wl.num_points, *d_uvw, d_vis, d_comp)
+            plan.results()
+        fg.device_synchronize()
+        stage = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.grid(wl.num_points, *d_uvw, d_vis, d_comp)
+            plan.results()
+            stage.append(plan.timings())
+        fg.device_synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        out[name] = {"value": wl.num_points / (ms * 1e-3) / 1e6,
+                     "unit": "Mvis/s", "ms_per_step": ms,
+                     "scatter_ms": float(np.mean([s[3] for s in stage])),
+                     "storage": f"{name} taps and visibilities, FP32 products,"
+                                " FP64 index math, compensated two-float grid"}
+        plan.close()
+        d_vis.close()
+        d_comp.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +192,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline and the oracle error sample")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP16 / BF16 plan timings")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiler calibration: copy this many bytes once "
                          "with the library's k_copy kernel and exit")
@@ -242,6 +288,8 @@ def main():
                                 "FP32 157.3 TFLOP/s",
                      "flops_per_launch": flops},
     }
+    if not args.no_variants:
+        out["precision_variants"] = precision_variants(wl, d_uvw, args)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         out["cpu_baseline"] = {k: cb[k] for k in

@@ -33,10 +33,14 @@ EXPORTED_SYMBOLS = (
     "oskar_grid_wproj_last_error", "gw_error_string", "gw_last_error_detail",
     "gw_release_cached",
     "gw_plan_create", "gw_plan_free", "gw_plan_set_stream", "gw_grid_device",
-    "gw_plan_results", "gw_plan_timings",
+    "gw_plan_results", "gw_plan_timings", "gw_plan_set_tuning",
+    "gw_plan_get_tuning", "gw_plan_autotune",
     "gw_device_count", "gw_device_malloc", "gw_device_free",
     "gw_memcpy_h2d", "gw_memcpy_d2h", "gw_memset_device",
     "gw_device_synchronize", "gw_device_copy", "gw_compensated_resolve",
+    "gw_batch_create", "gw_batch_free", "gw_batch_add", "gw_batch_clear",
+    "gw_batch_set_stream", "gw_batch_run", "gw_batch_capture",
+    "gw_batch_launch", "gw_batch_results", "gw_batch_last_ms",
 )
 
 
@@ -110,6 +114,13 @@ def lib():
     L.gw_plan_results.restype = None
     L.gw_plan_timings.argtypes = [_vp, _dp, ct.c_int]
     L.gw_plan_timings.restype = ct.c_int
+    L.gw_plan_set_tuning.argtypes = [_vp, ct.c_int, ct.c_int, ct.c_int, _ip]
+    L.gw_plan_set_tuning.restype = None
+    L.gw_plan_get_tuning.argtypes = [_vp, ct.POINTER(ct.c_int), ct.c_int]
+    L.gw_plan_get_tuning.restype = ct.c_int
+    L.gw_plan_autotune.argtypes = [_vp, _sz, _vp, _vp, _vp, _vp, _vp,
+                                   ct.c_int, _dp, _ip]
+    L.gw_plan_autotune.restype = None
     L.gw_device_count.argtypes = []
     L.gw_device_count.restype = ct.c_int
     L.gw_device_malloc.argtypes = [_sz, _ip]
@@ -127,6 +138,24 @@ def lib():
     L.gw_device_copy.restype = None
     L.gw_compensated_resolve.argtypes = [_vp, _sz, _vp, _vp, _ip]
     L.gw_compensated_resolve.restype = None
+    L.gw_batch_create.argtypes = [_ip]
+    L.gw_batch_create.restype = _vp
+    L.gw_batch_free.argtypes = [_vp]
+    L.gw_batch_free.restype = None
+    L.gw_batch_add.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _ip]
+    L.gw_batch_add.restype = ct.c_int
+    L.gw_batch_clear.argtypes = [_vp]
+    L.gw_batch_clear.restype = None
+    L.gw_batch_set_stream.argtypes = [_vp, _vp]
+    L.gw_batch_set_stream.restype = None
+    for name in ("gw_batch_run", "gw_batch_capture", "gw_batch_launch"):
+        getattr(L, name).argtypes = [_vp, _ip]
+        getattr(L, name).restype = None
+    L.gw_batch_results.argtypes = [_vp, ct.c_int, _szp, _dp,
+                                   ct.POINTER(ct.c_ulonglong), _ip]
+    L.gw_batch_results.restype = None
+    L.gw_batch_last_ms.argtypes = [_vp]
+    L.gw_batch_last_ms.restype = ct.c_double
     _lib = L
     return L
 
@@ -428,10 +457,98 @@ class Plan:
         n = lib().gw_plan_timings(self.ptr, buf, 8)
         return list(buf[:n])
 
+    def set_tuning(self, tile_log2=-1, slot_cap=0, min_share=0):
+        st = ct.c_int(0)
+        lib().gw_plan_set_tuning(self.ptr, tile_log2, slot_cap, min_share,
+                                 ct.byref(st))
+        _check(st, "gw_plan_set_tuning")
+
+    def tuning(self):
+        """dict(tile_log2, slot_cap, min_share, region, max_tile_log2)."""
+        buf = (ct.c_int * 5)()
+        n = lib().gw_plan_get_tuning(self.ptr, buf, 5)
+        keys = ("tile_log2", "slot_cap", "min_share", "region",
+                "max_tile_log2")
+        return dict(zip(keys[:n], list(buf[:n])))
+
+    def autotune(self, n, d_uu, d_vv, d_ww, d_vis, d_scratch, reps=2):
+        """Keeps the fastest scatter shape for these points; returns its
+        device ms per call."""
+        st = ct.c_int(0)
+        ms = ct.c_double(0)
+        lib().gw_plan_autotune(self.ptr, n, _ptr(d_uu), _ptr(d_vv),
+                               _ptr(d_ww), _ptr(d_vis), _ptr(d_scratch),
+                               reps, ct.byref(ms), ct.byref(st))
+        _check(st, "gw_plan_autotune")
+        return ms.value
+
     def close(self):
         if getattr(self, "ptr", None):
             lib().gw_plan_free(self.ptr)
             self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many FP32 jobs gridded by one pipeline with one persistent scatter
+    launch; capture() records it as a HIP graph, launch() replays it."""
+
+    def __init__(self):
+        st = ct.c_int(0)
+        self.ptr = lib().gw_batch_create(ct.byref(st))
+        _check(st, "gw_batch_create")
+        self._keep = []
+
+    def add(self, plan, n, d_uu, d_vv, d_ww, d_vis, d_grid):
+        st = ct.c_int(0)
+        j = lib().gw_batch_add(self.ptr, plan.ptr, n, _ptr(d_uu), _ptr(d_vv),
+                               _ptr(d_ww), _ptr(d_vis), _ptr(d_grid),
+                               ct.byref(st))
+        _check(st, "gw_batch_add")
+        self._keep.append((plan, d_uu, d_vv, d_ww, d_vis, d_grid))
+        return j
+
+    def clear(self):
+        lib().gw_batch_clear(self.ptr)
+        self._keep = []
+
+    def _call(self, name):
+        st = ct.c_int(0)
+        getattr(lib(), name)(self.ptr, ct.byref(st))
+        _check(st, name)
+
+    def run(self):
+        self._call("gw_batch_run")
+
+    def capture(self):
+        self._call("gw_batch_capture")
+
+    def launch(self):
+        self._call("gw_batch_launch")
+
+    def results(self, job):
+        st = ct.c_int(0)
+        sk = ct.c_size_t(0)
+        nd = ct.c_double(0)
+        nu = ct.c_ulonglong(0)
+        lib().gw_batch_results(self.ptr, job, ct.byref(sk), ct.byref(nd),
+                               ct.byref(nu), ct.byref(st))
+        _check(st, "gw_batch_results")
+        return int(sk.value), float(nd.value), int(nu.value)
+
+    def last_ms(self):
+        return lib().gw_batch_last_ms(self.ptr)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().gw_batch_free(self.ptr)
+            self.ptr = None
+            self._keep = []
 
     def __del__(self):
         try:

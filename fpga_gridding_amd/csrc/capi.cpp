@@ -290,6 +290,11 @@ struct gw_plan
     gw::Plan* impl;
 };
 
+struct gw_batch
+{
+    gw::Batch* impl;
+};
+
 extern "C" {
 
 GW_EXPORT
@@ -493,6 +498,50 @@ void gw_grid_device(gw_plan* plan, size_t num_points, const void* d_uu,
 }
 
 GW_EXPORT
+void gw_plan_set_tuning(gw_plan* plan, int tile_log2, int slot_cap,
+        int min_share, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!plan) gw::fail(gw::kErrInvalidArgument, "plan is NULL");
+        plan->impl->configure(tile_log2, slot_cap, min_share);
+    });
+}
+
+GW_EXPORT
+int gw_plan_get_tuning(gw_plan* plan, int* values, int max_n)
+{
+    if (!plan || !values || max_n <= 0) return 0;
+    const int v[5] = {plan->impl->tile_bits(), plan->impl->slot_cap(),
+                      plan->impl->min_share(), plan->impl->region(),
+                      plan->impl->max_tile_bits()};
+    const int k = max_n < 5 ? max_n : 5;
+    for (int i = 0; i < k; ++i) values[i] = v[i];
+    return k;
+}
+
+GW_EXPORT
+void gw_plan_autotune(gw_plan* plan, size_t num_points, const void* d_uu,
+        const void* d_vv, const void* d_ww, const void* d_vis,
+        void* d_scratch_grid, int reps, double* best_ms, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!plan) gw::fail(gw::kErrInvalidArgument, "plan is NULL");
+        if (!d_scratch_grid || (num_points && (!d_uu || !d_vv || !d_ww ||
+                !d_vis)))
+            gw::fail(gw::kErrInvalidArgument, "device pointers must not be NULL");
+        const double ms = plan->impl->autotune(num_points, d_uu, d_vv, d_ww,
+                d_vis, d_scratch_grid, reps);
+        if (best_ms) *best_ms = ms;
+    });
+}
+
+GW_EXPORT
 void gw_plan_results(gw_plan* plan, size_t* num_skipped, double* norm_delta,
         unsigned long long* num_updates, int* status)
 {
@@ -550,6 +599,8 @@ GW_EXPORT void gw_memcpy_h2d(void* dst, const void* src, size_t bytes,
     if (!status) status = &local;
     if (*status) return;
     guarded(status, [&] {
+        gw::check_hip(hipDeviceSynchronize(), gw::kErrKernelLaunch,
+                "hipDeviceSynchronize");
         gw::check_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice),
                 gw::kErrMemCopy, "hipMemcpy H2D");
     });
@@ -562,6 +613,8 @@ GW_EXPORT void gw_memcpy_d2h(void* dst, const void* src, size_t bytes,
     if (!status) status = &local;
     if (*status) return;
     guarded(status, [&] {
+        gw::check_hip(hipDeviceSynchronize(), gw::kErrKernelLaunch,
+                "hipDeviceSynchronize");
         gw::check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost),
                 gw::kErrMemCopy, "hipMemcpy D2H");
     });
@@ -574,9 +627,118 @@ GW_EXPORT void gw_memset_device(void* ptr, int value, size_t bytes,
     if (!status) status = &local;
     if (*status) return;
     guarded(status, [&] {
+        gw::check_hip(hipDeviceSynchronize(), gw::kErrKernelLaunch,
+                "hipDeviceSynchronize");
         gw::check_hip(hipMemset(ptr, value, bytes), gw::kErrMemCopy,
                 "hipMemset");
+        gw::check_hip(hipDeviceSynchronize(), gw::kErrKernelLaunch,
+                "hipDeviceSynchronize");
     });
+}
+
+GW_EXPORT gw_batch* gw_batch_create(int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return nullptr;
+    gw_batch* out = nullptr;
+    guarded(status, [&] {
+        std::unique_ptr<gw::Batch> b(new gw::Batch());
+        out = new gw_batch{b.release()};
+    });
+    return out;
+}
+
+GW_EXPORT void gw_batch_free(gw_batch* batch)
+{
+    if (!batch) return;
+    delete batch->impl;
+    delete batch;
+}
+
+GW_EXPORT int gw_batch_add(gw_batch* batch, gw_plan* plan, size_t num_points,
+        const float* d_uu, const float* d_vv, const float* d_ww,
+        const float* d_vis, float* d_grid, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return -1;
+    int job = -1;
+    guarded(status, [&] {
+        if (!batch || !plan)
+            gw::fail(gw::kErrInvalidArgument, "batch and plan must not be NULL");
+        job = batch->impl->add(plan->impl, num_points, d_uu, d_vv, d_ww,
+                               d_vis, d_grid);
+    });
+    return job;
+}
+
+GW_EXPORT void gw_batch_clear(gw_batch* batch)
+{
+    if (batch) batch->impl->clear();
+}
+
+GW_EXPORT void gw_batch_set_stream(gw_batch* batch, void* hip_stream)
+{
+    if (!batch) return;
+    int status = 0;
+    guarded(&status, [&] {
+        batch->impl->set_stream(static_cast<hipStream_t>(hip_stream));
+    });
+}
+
+GW_EXPORT void gw_batch_run(gw_batch* batch, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!batch) gw::fail(gw::kErrInvalidArgument, "batch is NULL");
+        batch->impl->run();
+    });
+}
+
+GW_EXPORT void gw_batch_capture(gw_batch* batch, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!batch) gw::fail(gw::kErrInvalidArgument, "batch is NULL");
+        batch->impl->capture();
+    });
+}
+
+GW_EXPORT void gw_batch_launch(gw_batch* batch, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!batch) gw::fail(gw::kErrInvalidArgument, "batch is NULL");
+        batch->impl->launch();
+    });
+}
+
+GW_EXPORT void gw_batch_results(gw_batch* batch, int job, size_t* num_skipped,
+        double* norm_delta, unsigned long long* num_updates, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!batch) gw::fail(gw::kErrInvalidArgument, "batch is NULL");
+        batch->impl->results(job, num_skipped, norm_delta, num_updates);
+    });
+}
+
+GW_EXPORT double gw_batch_last_ms(gw_batch* batch)
+{
+    if (!batch) return 0.0;
+    int status = 0;
+    double ms = 0.0;
+    guarded(&status, [&] { ms = batch->impl->last_ms(); });
+    return ms;
 }
 
 GW_EXPORT void gw_compensated_resolve(const float* d_comp, size_t num_cells,

@@ -100,13 +100,57 @@ __device__ __forceinline__ unsigned long long spread_bits16(unsigned x)
     return v;
 }
 
+// Global-memory views of pointers whose address space the compiler cannot
+// see (loaded from memory rather than passed as kernel arguments), so that
+// accesses through them are global rather than flat instructions.
+template <typename U>
+using GlobalPtr = __attribute__((address_space(1))) U*;
+
+template <typename U>
+__device__ __forceinline__ GlobalPtr<U> as_global(U* p)
+{
+    return (GlobalPtr<U>)(uintptr_t)p;
+}
+
+// Hardware FP atomic add (no return) on global memory.
 __device__ __forceinline__ void atomic_add_real(float* p, float v)
 {
-    unsafeAtomicAdd(p, v);
+    (void)__builtin_amdgcn_global_atomic_fadd_f32(as_global(p), v);
 }
 __device__ __forceinline__ void atomic_add_real(double* p, double v)
 {
-    unsafeAtomicAdd(p, v);
+    (void)__builtin_amdgcn_global_atomic_fadd_f64(as_global(p), v);
+}
+
+// A stored tap pair (TapT) at byte address p, from global memory.
+__device__ __forceinline__ float2 ld_tap(const float2*, const char* p)
+{
+    const GlobalPtr<const float> q = as_global(
+            reinterpret_cast<const float*>(p));
+    return make_float2(q[0], q[1]);
+}
+__device__ __forceinline__ double2 ld_tap(const double2*, const char* p)
+{
+    const GlobalPtr<const double> q = as_global(
+            reinterpret_cast<const double*>(p));
+    return make_double2(q[0], q[1]);
+}
+__device__ __forceinline__ float2 ld_tap(const HalfPair*, const char* p)
+{
+    const GlobalPtr<const unsigned> q = as_global(
+            reinterpret_cast<const unsigned*>(p));
+    const unsigned u = q[0];
+    HalfPair h;
+    __builtin_memcpy(&h, &u, 4);
+    return tap_pair(h);
+}
+__device__ __forceinline__ float2 ld_tap(const BF16Pair*, const char* p)
+{
+    const GlobalPtr<const unsigned> q = as_global(
+            reinterpret_cast<const unsigned*>(p));
+    const unsigned u = q[0];
+    return make_float2(bf16_to_float((unsigned short)(u & 0xFFFFu)),
+                       bf16_to_float((unsigned short)(u >> 16)));
 }
 
 // One component of a compensated two-float grid cell, p[0] = hi and
@@ -227,7 +271,8 @@ __global__ void __launch_bounds__(256) k_locate(
         unsigned* __restrict__ combo, Summary* __restrict__ summary,
         unsigned* __restrict__ gw_out,
         unsigned long long* __restrict__ diag_support,
-        unsigned long long* __restrict__ diag_plane, int diag_plane_cap)
+        unsigned long long* __restrict__ diag_plane, int diag_plane_cap,
+        unsigned long long key_tag, unsigned idx_base)
 {
 #pragma clang fp contract(off)
     __shared__ unsigned s_combo[kComboLds];
@@ -313,8 +358,8 @@ __global__ void __launch_bounds__(256) k_locate(
             r.im = to_real(vis[2 * i + 1]);
             rec[i] = r;
         }
-        keys[i] = key;
-        idx[i] = (unsigned)i;
+        keys[i] = key | key_tag;
+        idx[i] = idx_base + (unsigned)i;
     }
 
     // Block reductions: one atomic per block for the counters and box.
@@ -661,6 +706,54 @@ struct GridLds
 constexpr unsigned kImBytes = kRegionCells * sizeof(double);
 static_assert(kImBytes < 65536, "imaginary offset must fit a DS offset");
 
+// Read-only device data loaded through the constant address space: the
+// compiler may then fetch uniform records with scalar loads even where
+// other pointers (the grid) are written in the same loop.  Field-wise, so
+// that the host compilation pass, which has no such address space for
+// class types, accepts it.
+template <typename U>
+using ConstPtr = const __attribute__((address_space(4))) U*;
+
+template <typename U>
+__device__ __forceinline__ ConstPtr<U> as_const(const void* p)
+{
+    return (ConstPtr<U>)(uintptr_t)p;
+}
+
+__device__ __forceinline__ int4 ld_meta(const int4* p, int i)
+{
+    const ConstPtr<int> q = as_const<int>(p + i);
+    return make_int4(q[0], q[1], q[2], q[3]);
+}
+
+template <typename T4>
+__device__ __forceinline__ T4 ld_vq(const T4* p, int i)
+{
+    typedef decltype(T4().x) T;
+    const ConstPtr<T> q = as_const<T>(p + i);
+    T4 r;
+    r.x = q[0];
+    r.y = q[1];
+    r.z = q[2];
+    r.w = q[3];
+    return r;
+}
+
+__device__ __forceinline__ PlaneInfo ld_pinfo(const PlaneInfo* p, int i)
+{
+    const ConstPtr<int> q = as_const<int>(p + i);
+    PlaneInfo r;
+    r.W = q[0];
+    r.S = q[1];
+    r.wpp = q[2];
+    r.nslot = q[3];
+    r.wmagic = (unsigned)q[4];
+    r.sshift = q[5];
+    r.eoff = (unsigned long long)(unsigned)q[6] |
+            ((unsigned long long)(unsigned)q[7] << 32);
+    return r;
+}
+
 template <typename T>
 struct ScatterCtx
 {
@@ -767,8 +860,8 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
 #pragma unroll
         for (int s = 0; s < NS; ++s)
         {
-            const auto t = tap_pair(
-                    *reinterpret_cast<const TapT*>(tp + toff[s]));
+            const auto t = ld_tap(static_cast<const TapT*>(nullptr),
+                    tp + toff[s]);
             nre[s] = (T)t.x;
             nim[s] = (T)t.y;
         }
@@ -781,8 +874,8 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
             tim[s] = (A)nim[s];
         }
     };
-    int4 m = x.meta[a];
-    T4 v = x.vq[a];
+    int4 m = ld_meta(x.meta, a);
+    T4 v = ld_vq(x.vq, a);
     load_taps(m.y);
     if (LDS)
     {
@@ -828,8 +921,8 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
     T4 vn = v;
     if (a + 1 < b)
     {
-        mn = x.meta[a + 1];
-        vn = x.vq[a + 1];
+        mn = ld_meta(x.meta, a + 1);
+        vn = ld_vq(x.vq, a + 1);
     }
     unsigned base = origin(m.x);
     // The first table's taps are converted before the loop, so the loads
@@ -843,8 +936,8 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
         T4 vnn = vn;
         if (k + 2 < b)
         {
-            mnn = x.meta[k + 2];
-            vnn = x.vq[k + 2];
+            mnn = ld_meta(x.meta, k + 2);
+            vnn = ld_vq(x.vq, k + 2);
         }
         const bool new_table = more && mn.y != m.y;
         if (new_table) load_taps(mn.y);
@@ -995,6 +1088,125 @@ __device__ __forceinline__ void flush_region_comp(GridLds& L, int R, int y0,
     }
 }
 
+// One work item (up to kJobPoints sorted points of one tile, origin
+// job.x = u | v << 16, points [job.y, job.y + job.z)) by one workgroup:
+// plane segments found with a block scan, streams of waves per segment, and
+// with an LDS region (lds) the region added to the grid at the end and
+// cleared, sink cells included (the next item may have another R).
+template <typename T, bool DIAG, typename TapT, bool COMP>
+__device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
+        const PlaneInfo* __restrict__ pinfo, const int4& job, bool lds,
+        int min_share)
+{
+    const int tid = threadIdx.x;
+    const int NT = blockDim.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int R = x.R, G = x.G, row0 = x.row0;
+    T* const grid = x.grid;
+    const int4* __restrict__ meta = x.meta;
+    x.ux0 = job.x & 0xFFFF;
+    x.vy0 = (int)((unsigned)job.x >> 16);
+    const int start = job.y;
+    const int len = job.z;
+    // Plane segments of the job (points are sorted by plane).
+    unsigned heads = 0;
+    int myplane[2] = {0, 0};
+    bool myhead[2] = {false, false};
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+    {
+        const int i = 2 * tid + q;
+        if (i < len)
+        {
+            myplane[q] = meta[start + i].z;
+            myhead[q] = (i == 0) || meta[start + i - 1].z != myplane[q];
+            heads += myhead[q] ? 1u : 0u;
+        }
+    }
+    unsigned nseg;
+    unsigned pos = block_exclusive_scan(heads, L.tmp, &nseg);
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+    {
+        if (myhead[q])
+        {
+            L.seg[pos] = (unsigned)(2 * tid + q);
+            L.segp[pos] = myplane[q];
+            ++pos;
+        }
+    }
+    if (tid == 0)
+    {
+        L.seg[nseg] = (unsigned)len;
+        L.nseg = nseg;
+    }
+    __syncthreads();
+    // Values read back from LDS are the same in every lane; make that
+    // explicit so that the point loop runs on scalar registers and scalar
+    // loads.
+    nseg = (unsigned)__builtin_amdgcn_readfirstlane((int)L.nseg);
+    for (unsigned g = 0; g < nseg; ++g)
+    {
+        const int plane = __builtin_amdgcn_readfirstlane(L.segp[g]);
+        const PlaneInfo pi = ld_pinfo(pinfo, plane);
+        const int ra = __builtin_amdgcn_readfirstlane((int)L.seg[g]);
+        const int rb = __builtin_amdgcn_readfirstlane((int)L.seg[g + 1]);
+        // streams = nwaves / wpp, a power of two (2^sshift).
+        const int smask = (1 << pi.sshift) - 1;
+        const int stream = wave / pi.wpp;
+        if (stream > smask) continue;
+        const int wis = wave - stream * pi.wpp;
+        const int n = rb - ra;
+        const int share = max(min_share, (n + smask) >> pi.sshift);
+        const int q = (stream - ra) & smask;
+        const int sa = start + ra + q * share;
+        const int sb = min(start + rb, sa + share);
+        if (sa >= sb) continue;
+        const int pass = 64 * pi.wpp * pi.nslot;
+        for (int cb = 0; cb < pi.W * pi.W; cb += pass)
+        {
+            if (lds && !DIAG)
+                scatter_dispatch<T, DIAG, true, TapT, COMP>(x, pi, wis, sa,
+                        sb, cb);
+            else
+                scatter_dispatch<T, DIAG, false, TapT, COMP>(x, pi, wis, sa,
+                        sb, cb);
+        }
+    }
+    __syncthreads();
+    if (lds)
+    {
+        // Region -> grid, coalesced along rows, clearing it for the next
+        // item.  Cells no footprint reached hold zero and are skipped (they
+        // may lie off the grid).
+        if constexpr (COMP)
+            flush_region_comp(L, R, x.vy0 - row0, x.ux0, G,
+                              reinterpret_cast<float*>(grid));
+        else
+            for (int i = tid; i < R * R; i += NT)
+            {
+                const double re = L.re[i];
+                const double im = L.im[i];
+                if (re != 0.0 || im != 0.0)
+                {
+                    L.re[i] = 0.0;
+                    L.im[i] = 0.0;
+                    const int y = i / R;
+                    const int xx = i - y * R;
+                    const size_t g = (size_t)(x.vy0 - row0 + y) * (size_t)G +
+                            (size_t)(x.ux0 + xx);
+                    grid_add<T, COMP>(grid, g, re, im);
+                }
+            }
+        for (int i = tid; i < kSinkCells; i += NT)
+        {
+            L.re[R * R + i] = 0.0;
+            L.im[R * R + i] = 0.0;
+        }
+        __syncthreads();
+    }
+}
+
 // Grid-stride loop over the jobs (persistent: the plan launches as many
 // workgroups as fit the device at once).  Taps are stored as TapT pairs;
 // COMP selects the compensated two-float grid layout.
@@ -1012,7 +1224,6 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
     __shared__ GridLds L;
     const int tid = threadIdx.x;
     const int NT = blockDim.x;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool lds = !force_direct && R > 0;
     ScatterCtx<T> x;
     x.meta = meta;
@@ -1033,105 +1244,8 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
     }
     const unsigned total = summary->total_items;
     for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
-    {
-        const int4 job = items[it];
-        x.ux0 = job.x & 0xFFFF;
-        x.vy0 = (int)((unsigned)job.x >> 16);
-        const int start = job.y;
-        const int len = job.z;
-        // Plane segments of the job (points are sorted by plane).
-        unsigned heads = 0;
-        int myplane[2] = {0, 0};
-        bool myhead[2] = {false, false};
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-        {
-            const int i = 2 * tid + q;
-            if (i < len)
-            {
-                myplane[q] = meta[start + i].z;
-                myhead[q] = (i == 0) || meta[start + i - 1].z != myplane[q];
-                heads += myhead[q] ? 1u : 0u;
-            }
-        }
-        unsigned nseg;
-        unsigned pos = block_exclusive_scan(heads, L.tmp, &nseg);
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-        {
-            if (myhead[q])
-            {
-                L.seg[pos] = (unsigned)(2 * tid + q);
-                L.segp[pos] = myplane[q];
-                ++pos;
-            }
-        }
-        if (tid == 0)
-        {
-            L.seg[nseg] = (unsigned)len;
-            L.nseg = nseg;
-        }
-        __syncthreads();
-        // Values read back from LDS are the same in every lane; make that
-        // explicit so that the point loop runs on scalar registers and
-        // scalar loads.
-        nseg = (unsigned)__builtin_amdgcn_readfirstlane((int)L.nseg);
-        for (unsigned g = 0; g < nseg; ++g)
-        {
-            const int plane = __builtin_amdgcn_readfirstlane(L.segp[g]);
-            const PlaneInfo pi = pinfo[plane];
-            const int ra = __builtin_amdgcn_readfirstlane((int)L.seg[g]);
-            const int rb = __builtin_amdgcn_readfirstlane((int)L.seg[g + 1]);
-            // streams = nwaves / wpp, a power of two (2^sshift).
-            const int smask = (1 << pi.sshift) - 1;
-            const int stream = wave / pi.wpp;
-            if (stream > smask) continue;
-            const int wis = wave - stream * pi.wpp;
-            const int n = rb - ra;
-            const int share = max(min_share, (n + smask) >> pi.sshift);
-            const int q = (stream - ra) & smask;
-            const int sa = start + ra + q * share;
-            const int sb = min(start + rb, sa + share);
-            if (sa >= sb) continue;
-            const int pass = 64 * pi.wpp * pi.nslot;
-            for (int cb = 0; cb < pi.W * pi.W; cb += pass)
-            {
-                if (lds && !DIAG)
-                    scatter_dispatch<T, DIAG, true, TapT, COMP>(x, pi, wis, sa,
-                            sb, cb);
-                else
-                    scatter_dispatch<T, DIAG, false, TapT, COMP>(x, pi, wis, sa,
-                            sb, cb);
-            }
-        }
-        __syncthreads();
-        if (lds)
-        {
-            // Region -> grid, coalesced along rows, clearing it for the next
-            // job.  Cells no footprint reached hold zero and are skipped
-            // (they may lie off the grid).
-            if constexpr (COMP)
-                flush_region_comp(L, R, x.vy0 - row0, x.ux0, G,
-                                  reinterpret_cast<float*>(grid));
-            else
-                for (int i = tid; i < R * R; i += NT)
-                {
-                    const double re = L.re[i];
-                    const double im = L.im[i];
-                    if (re != 0.0 || im != 0.0)
-                    {
-                        L.re[i] = 0.0;
-                        L.im[i] = 0.0;
-                        const int y = i / R;
-                        const int xx = i - y * R;
-                        const size_t g = (size_t)(x.vy0 - row0 + y) *
-                                (size_t)G + (size_t)(x.ux0 + xx);
-                        grid_add<T, COMP>(grid, g, re, im);
-                    }
-                }
-            __syncthreads();
-        }
-    }
+        grid_work_item<T, DIAG, TapT, COMP>(L, x, pinfo, items[it], lds,
+                                            min_share);
 }
 
 // -------------------------------------------------------------------------
@@ -1345,62 +1459,16 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     combo_bits_ = std::max(1, bits_for((unsigned long long)(K_ * K_ - 1)));
     plane_bits_ = bits_plane_((int)P_);
 
-    // Tile side T = 2^t (<= 64): the largest whose LDS region
-    // (T + Wmax - 1)^2 plus the sink cells, real and imaginary, fits the
-    // budget.  If none does the scatter adds straight into the grid in HBM.
-    const long long wmax = 2ll * s_max_ + 1;
-    region_ = 0;
-    tile_bits_ = std::min(6, coord_bits_);
-    for (int t = std::min(6, coord_bits_); t >= 0; --t)
-    {
-        const long long R = (1ll << t) + wmax - 1;
-        if (R * R + kSinkCells <= kRegionCells)
-        {
-            tile_bits_ = t;
-            region_ = (int)R;
-            break;
-        }
-    }
-    tile_shift_ = 2 * tile_bits_ + combo_bits_ + plane_bits_;
     key_bits_ = 2 * coord_bits_ + combo_bits_ + plane_bits_;
-
-    // Streams per plane: wpp waves cover the footprint with <= smax cells
-    // per lane (larger footprints take several passes).
-    const int smax = precision == kPrecDouble ? slots_max<double>() :
-            slots_max<float>();
     pinfo_.resize(P_);
     for (size_t p = 0; p < P_; ++p)
     {
         PlaneInfo& pi = pinfo_[p];
         pi.S = support_[p];
         pi.W = 2 * pi.S + 1;
-        const long long cells = (long long)pi.W * pi.W;
-        pi.wpp = 1;
-        while (pi.wpp < kGridThreads / 64 &&
-                (long long)pi.wpp * 64 * smax < cells)
-            pi.wpp *= 2;
-        pi.nslot = (int)std::min<long long>(smax,
-                (cells + 64ll * pi.wpp - 1) / (64ll * pi.wpp));
-        pi.sshift = 0;
-        while ((kGridThreads / 64 / pi.wpp) >> (pi.sshift + 1)) ++pi.sshift;
-        // ceil(2^32 / W) divides exactly over c < W^2 when
-        // (W^2 - 1) * (m * W - 2^32) < 2^32 (W == 1: only c == 0 occurs).
-        const unsigned long long two32 = 1ull << 32;
-        const unsigned long long wm = pi.W == 1 ? 0xFFFFFFFFull :
-                (two32 + (unsigned long long)pi.W - 1) / (unsigned long long)pi.W;
-        const unsigned long long err = wm * (unsigned long long)pi.W - two32;
-        pi.wmagic = (pi.W == 1 ||
-                (unsigned long long)(cells - 1) * err < two32) ?
-                (unsigned)wm : 0u;
         pi.eoff = 0;
     }
-
-    // Tuning override for sweeps: GW_SCATTER_MIN_SHARE=<1..2048>.
-    if (const char* e = std::getenv("GW_SCATTER_MIN_SHARE"))
-    {
-        const int v = std::atoi(e);
-        if (v >= 1 && v <= kJobPoints) min_share_ = v;
-    }
+    configure(-1, 0, 0);
 
     check_hip(hipGetDevice(&device_), kErrComputeDevices, "hipGetDevice");
     check_hip(hipDeviceGetAttribute(&n_cu_,
@@ -1496,6 +1564,131 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     check_hip(hipMemcpy(d_normtab_, normtab.data(),
             normtab.size() * sizeof(double), hipMemcpyHostToDevice),
             kErrMemCopy, "upload norm table");
+}
+
+int Plan::max_tile_bits() const
+{
+    const long long wmax = 2ll * s_max_ + 1;
+    for (int t = std::min(6, coord_bits_); t >= 0; --t)
+    {
+        const long long R = (1ll << t) + wmax - 1;
+        if (R * R + kSinkCells <= kRegionCells) return t;
+    }
+    return -1;
+}
+
+void Plan::configure(int tile_bits, int slot_cap, int min_share)
+{
+    const int smax_all = prec_ == kPrecDouble ? slots_max<double>() :
+            slots_max<float>();
+    if (slot_cap < 0 || slot_cap > smax_all)
+        fail(kErrInvalidArgument, "slot_cap must be in [0, " +
+             std::to_string(smax_all) + "] (0: the largest)");
+    if (min_share < 0 || min_share > kJobPoints)
+        fail(kErrInvalidArgument, "min_share must be in [0, " +
+             std::to_string(kJobPoints) + "] (0: the default)");
+    // Tile side T = 2^t (<= 64) of footprint origins: by default the
+    // largest whose LDS region (T + Wmax - 1)^2 plus the sink cells fits
+    // each region array.  If none does the scatter adds straight into the
+    // grid in HBM.
+    const int tmax = max_tile_bits();
+    const long long wmax = 2ll * s_max_ + 1;
+    if (tile_bits < 0)
+    {
+        tile_bits_ = tmax >= 0 ? tmax : std::min(6, coord_bits_);
+        region_ = tmax >= 0 ? (int)((1ll << tmax) + wmax - 1) : 0;
+    }
+    else
+    {
+        if (tile_bits > std::min(6, coord_bits_))
+            fail(kErrInvalidArgument, "tile_log2 must be in [0, " +
+                 std::to_string(std::min(6, coord_bits_)) + "]");
+        tile_bits_ = tile_bits;
+        const long long R = (1ll << tile_bits) + wmax - 1;
+        region_ = R * R + kSinkCells <= kRegionCells ? (int)R : 0;
+    }
+    tile_shift_ = 2 * tile_bits_ + combo_bits_ + plane_bits_;
+    slot_cap_ = slot_cap > 0 ? slot_cap : smax_all;
+    min_share_ = min_share > 0 ? min_share : kMinShare;
+
+    // Streams per plane: wpp waves cover the footprint with <= slot_cap_
+    // cells per lane (larger footprints take several passes).
+    const int smax = slot_cap_;
+    for (size_t p = 0; p < P_; ++p)
+    {
+        PlaneInfo& pi = pinfo_[p];
+        const long long cells = (long long)pi.W * pi.W;
+        pi.wpp = 1;
+        while (pi.wpp < kGridThreads / 64 &&
+                (long long)pi.wpp * 64 * smax < cells)
+            pi.wpp *= 2;
+        pi.nslot = (int)std::min<long long>(smax,
+                (cells + 64ll * pi.wpp - 1) / (64ll * pi.wpp));
+        pi.sshift = 0;
+        while ((kGridThreads / 64 / pi.wpp) >> (pi.sshift + 1)) ++pi.sshift;
+        // ceil(2^32 / W) divides exactly over c < W^2 when
+        // (W^2 - 1) * (m * W - 2^32) < 2^32 (W == 1: only c == 0 occurs).
+        const unsigned long long two32 = 1ull << 32;
+        const unsigned long long wm = pi.W == 1 ? 0xFFFFFFFFull :
+                (two32 + (unsigned long long)pi.W - 1) /
+                (unsigned long long)pi.W;
+        const unsigned long long err = wm * (unsigned long long)pi.W - two32;
+        pi.wmagic = (pi.W == 1 ||
+                (unsigned long long)(cells - 1) * err < two32) ?
+                (unsigned)wm : 0u;
+    }
+    if (d_pinfo_)
+    {
+        if (stream_)
+            check_hip(hipStreamSynchronize(stream_), kErrKernelLaunch,
+                      "stream synchronize");
+        check_hip(hipMemcpy(d_pinfo_, pinfo_.data(), P_ * sizeof(PlaneInfo),
+                hipMemcpyHostToDevice), kErrMemCopy, "upload plane table");
+    }
+}
+
+double Plan::autotune(size_t n, const void* uu, const void* vv,
+                      const void* ww, const void* vis, void* scratch_grid,
+                      int reps)
+{
+    if (reps < 1) reps = 1;
+    auto measure = [&]() {
+        double best = 1e300;
+        for (int r = 0; r < reps + 1; ++r)   // first run warms up
+        {
+            locate(n, uu, vv, ww, vis, nullptr);
+            scatter(n, scratch_grid, 0, nullptr);
+            double ms[5];
+            if (timings(ms, 5) == 5 && r > 0) best = std::min(best, ms[4]);
+        }
+        return best;
+    };
+    const int smax_all = prec_ == kPrecDouble ? slots_max<double>() :
+            slots_max<float>();
+    int bt = tile_bits_, bs = slot_cap_, bm = min_share_;
+    configure(bt, bs, bm);
+    double best = measure();
+    auto consider = [&](int t, int sc, int ms) {
+        configure(t, sc, ms);
+        const double v = measure();
+        if (v < best)
+        {
+            best = v;
+            bt = t;
+            bs = sc;
+            bm = ms;
+        }
+    };
+    const int tmax = max_tile_bits();
+    if (tmax >= 0)
+        for (int t = tmax; t >= std::max(0, tmax - 2); --t)
+            if (t != bt) consider(t, bs, bm);
+    for (int sc : {4, 6, 8})
+        if (sc <= smax_all && sc != bs) consider(bt, sc, bm);
+    for (int ms : {4, 8, 16})
+        if (ms != bm) consider(bt, bs, ms);
+    configure(bt, bs, bm);
+    return best;
 }
 
 Plan::~Plan()
@@ -1600,18 +1793,27 @@ void Plan::ensure_capacity(size_t n)
     cap_ = cap;
 }
 
-void Plan::scan_u32(const unsigned* in, unsigned* out, const unsigned* count,
-                    size_t max_n, unsigned* total)
+// Exclusive scan of *count (<= max_n) u32 values on `stream`; *total gets
+// their sum.  part holds max_n / kScanTile + 2 words.
+void scan_u32_enqueue(const unsigned* in, unsigned* out, const unsigned* count,
+                      size_t max_n, unsigned* total, unsigned* part,
+                      hipStream_t stream)
 {
     const unsigned nb = (unsigned)std::max<size_t>(1,
             (max_n + kScanTile - 1) / kScanTile);
-    hipLaunchKernelGGL(k_scan_up, dim3(nb), dim3(kScanThreads), 0, stream_,
-            in, count, d_scan_part_);
-    hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kScanThreads), 0, stream_,
-            d_scan_part_, count, total);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanThreads), 0, stream_,
-            in, count, (const unsigned*)d_scan_part_, out);
+    hipLaunchKernelGGL(k_scan_up, dim3(nb), dim3(kScanThreads), 0, stream,
+            in, count, part);
+    hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kScanThreads), 0, stream,
+            part, count, total);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanThreads), 0, stream,
+            in, count, (const unsigned*)part, out);
     check_hip(hipGetLastError(), kErrKernelLaunch, "scan launch");
+}
+
+void Plan::scan_u32(const unsigned* in, unsigned* out, const unsigned* count,
+                    size_t max_n, unsigned* total)
+{
+    scan_u32_enqueue(in, out, count, max_n, total, d_scan_part_, stream_);
 }
 
 template <typename T, typename TU, typename TV, typename Rec>
@@ -1635,7 +1837,7 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
                 d_support_, h_, K_, tile_bits_, combo_bits_, key_bits_,
                 d_keys_[0], d_idx_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, d_gw_, diag->per_support,
-                diag->per_plane, diag->plane_cap);
+                diag->per_plane, diag->plane_cap, 0ull, 0u);
     else
         hipLaunchKernelGGL((k_locate<T, false, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
@@ -1643,7 +1845,7 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
                 d_keys_[0], d_idx_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, (unsigned*)nullptr,
                 (unsigned long long*)nullptr, (unsigned long long*)nullptr,
-                0);
+                0, 0ull, 0u);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_locate launch");
     check_hip(hipEventRecord(ev_[1], stream_), kErrKernelLaunch, "event");
 }
@@ -1774,6 +1976,552 @@ int Plan::timings(double* ms, int max_n)
     const int n = std::min(max_n, 5);
     for (int i = 0; i < n; ++i) ms[i] = t[i];
     return n;
+}
+
+// -------------------------------------------------------------------------
+// Batched jobs: one pipeline, one persistent scatter launch
+// -------------------------------------------------------------------------
+//
+// Point keys carry their job index in bits 48-63 (the plan's own key below,
+// < 2^48).  The global tile index of a point is its job's tile_base plus
+// its plan-level tile (the key bits above tile_shift).
+
+constexpr int kBatchJobShift = 48;
+static_assert(offsetof(BatchJob, etab) == 8 && offsetof(BatchJob, grid) == 16
+        && offsetof(BatchJob, G) == 24 && offsetof(BatchJob, R) == 28 &&
+        offsetof(BatchJob, min_share) == 44, "BatchJob word layout");
+static_assert(sizeof(PlaneInfo) == 32 && offsetof(PlaneInfo, eoff) == 24,
+        "PlaneInfo word layout");
+constexpr unsigned long long kBatchLocalMask = (1ull << kBatchJobShift) - 1;
+
+// Points per global tile (skipped points, key == 1 << key_bits, excluded).
+__global__ void __launch_bounds__(256) k_tile_count(size_t n,
+        const unsigned long long* __restrict__ keys,
+        const BatchJob* __restrict__ jobs, unsigned* __restrict__ count)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+    {
+        const unsigned long long key = keys[i];
+        const BatchJob& d = jobs[key >> kBatchJobShift];
+        const unsigned long long local = key & kBatchLocalMask;
+        if (local >> d.key_bits) continue;
+        atomicAdd(&count[d.tile_base + (unsigned)(local >> d.tile_shift)], 1u);
+    }
+}
+
+// Counting-sort scatter: each point to its tile's range (the order inside a
+// tile is restored by k_item_sort).
+__global__ void __launch_bounds__(256) k_tile_scatter(size_t n,
+        const unsigned long long* __restrict__ keys,
+        const unsigned* __restrict__ idx, const BatchJob* __restrict__ jobs,
+        const unsigned* __restrict__ tstart, unsigned* __restrict__ cursor,
+        unsigned long long* __restrict__ keys_out,
+        unsigned* __restrict__ idx_out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+    {
+        const unsigned long long key = keys[i];
+        const BatchJob& d = jobs[key >> kBatchJobShift];
+        const unsigned long long local = key & kBatchLocalMask;
+        if (local >> d.key_bits) continue;
+        const unsigned t = d.tile_base + (unsigned)(local >> d.tile_shift);
+        const unsigned pos = tstart[t] + atomicAdd(&cursor[t], 1u);
+        keys_out[pos] = key;
+        idx_out[pos] = idx[i];
+    }
+}
+
+// Work items per tile: ceil(points / kJobPoints).
+__global__ void __launch_bounds__(256) k_tile_items(
+        const unsigned* __restrict__ ntiles, const unsigned* __restrict__ count,
+        unsigned* __restrict__ icnt)
+{
+    const unsigned nt = *ntiles;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < nt;
+         t += gridDim.x * blockDim.x)
+        icnt[t] = (count[t] + kJobPoints - 1) / kJobPoints;
+}
+
+// Work items = (tile origin u | v << 16, first sorted point, points, job).
+__global__ void __launch_bounds__(256) k_write_tile_items(unsigned ntiles,
+        const unsigned* __restrict__ count, const unsigned* __restrict__ tstart,
+        const unsigned* __restrict__ ioff, const BatchJob* __restrict__ jobs,
+        int njobs, unsigned items_cap, int4* __restrict__ items)
+{
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles;
+         t += gridDim.x * blockDim.x)
+    {
+        const unsigned c = count[t];
+        if (c == 0) continue;
+        // Job owning global tile t: the last with tile_base <= t.
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].tile_base <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchJob& d = jobs[lo];
+        const unsigned lt = t - d.tile_base;
+        const unsigned ux0 = compact_bits16(lt) << d.tile_bits;
+        const unsigned vy0 = compact_bits16((unsigned long long)lt >> 1)
+                << d.tile_bits;
+        const int origin = (int)(ux0 | (vy0 << 16));
+        unsigned o = ioff[t];
+        for (unsigned first = 0; first < c; first += kJobPoints, ++o)
+            if (o < items_cap)
+                items[o] = make_int4(origin, (int)(tstart[t] + first),
+                        (int)min((unsigned)kJobPoints, c - first), lo);
+    }
+}
+
+// Each work item's points sorted by key (job, tile, plane, position inside
+// the tile, sub-cell table) with a bitonic sort in LDS.
+__global__ void __launch_bounds__(1024) k_item_sort(
+        const Summary* __restrict__ bsum, const int4* __restrict__ items,
+        unsigned long long* __restrict__ keys, unsigned* __restrict__ idx)
+{
+    __shared__ unsigned long long sk[kJobPoints];
+    __shared__ unsigned si[kJobPoints];
+    const int tid = threadIdx.x;
+    const unsigned total = bsum->total_items;
+    for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
+    {
+        const int4 item = items[it];
+        const int start = item.y, len = item.z;
+        for (int i = tid; i < kJobPoints; i += blockDim.x)
+        {
+            sk[i] = i < len ? keys[start + i] : ~0ull;
+            si[i] = i < len ? idx[start + i] : 0u;
+        }
+        __syncthreads();
+        for (int k = 2; k <= kJobPoints; k <<= 1)
+        {
+            for (int j = k >> 1; j > 0; j >>= 1)
+            {
+                for (int i = tid; i < kJobPoints; i += blockDim.x)
+                {
+                    const int l = i ^ j;
+                    if (l > i)
+                    {
+                        const bool up = (i & k) == 0;
+                        const unsigned long long a = sk[i], b = sk[l];
+                        if ((a > b) == up)
+                        {
+                            sk[i] = b;
+                            sk[l] = a;
+                            const unsigned t = si[i];
+                            si[i] = si[l];
+                            si[l] = t;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = tid; i < len; i += blockDim.x)
+        {
+            keys[start + i] = sk[i];
+            idx[start + i] = si[i];
+        }
+        __syncthreads();
+    }
+}
+
+// Sorted point records of all jobs (as k_gather, with each point's job).
+__global__ void __launch_bounds__(256) k_gather_batch(
+        const Summary* __restrict__ bsum,
+        const unsigned long long* __restrict__ keys,
+        const unsigned* __restrict__ idx, const RecF* __restrict__ rec,
+        const BatchJob* __restrict__ jobs, int4* __restrict__ meta,
+        float4* __restrict__ vq)
+{
+    const unsigned nv = bsum->nv32;
+    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
+         s += gridDim.x * blockDim.x)
+    {
+        const BatchJob& d = jobs[keys[s] >> kBatchJobShift];
+        const RecF r = rec[idx[s]];
+        const int plane = (unsigned)r.sub >> 17;
+        const float sgn = ((r.sub >> 16) & 1) ? -1.0f : 1.0f;
+        const int off_u = (r.sub & 0xFF) - 128;
+        const int off_v = ((r.sub >> 8) & 0xFF) - 128;
+        const int table = (off_v + d.sub_h) * d.sub_k + (off_u + d.sub_h);
+        meta[s] = make_int4(r.pos, table, plane, 0);
+        vq[s] = make_float4(r.re, -sgn * r.im, r.im, sgn * r.re);
+    }
+}
+
+// The scatter of every job's work items, one item per workgroup at a time
+// (persistent); each item takes its job's tables, grid and region side.
+__global__ void __launch_bounds__(kGridThreads) k_grid_batch(
+        const Summary* __restrict__ bsum, const int4* __restrict__ items,
+        const BatchJob* __restrict__ jobs, const int4* __restrict__ meta,
+        const float4* __restrict__ vq)
+{
+    __shared__ GridLds L;
+    for (int i = threadIdx.x; i < kRegionCells; i += blockDim.x)
+    {
+        L.re[i] = 0.0;
+        L.im[i] = 0.0;
+    }
+    __syncthreads();
+    ScatterCtx<float> x;
+    x.meta = meta;
+    x.vq = vq;
+    x.lds = &L;
+    x.diag_count = nullptr;
+    x.diag_wsum = nullptr;
+    x.row0 = 0;
+    const unsigned total = bsum->total_items;
+    for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
+    {
+        const int4 item = items[it];
+        // BatchJob fields by word: pinfo, etab, grid (2 words each), G, R,
+        // tile_bits, tile_shift, key_bits, min_share, ...
+        const ConstPtr<unsigned> w = as_const<unsigned>(jobs + item.w);
+        auto ptr_at = [&](int k) {
+            return (uintptr_t)w[k] | ((uintptr_t)w[k + 1] << 32);
+        };
+        const PlaneInfo* pinfo = reinterpret_cast<const PlaneInfo*>(ptr_at(0));
+        x.etab = reinterpret_cast<const void*>(ptr_at(2));
+        x.grid = reinterpret_cast<float*>(ptr_at(4));
+        x.G = (int)w[6];
+        x.R = (int)w[7];
+        const int min_share = (int)w[11];
+        grid_work_item<float, false, float2, false>(L, x, pinfo, item,
+                x.R > 0, min_share);
+    }
+}
+
+Batch::Batch()
+{
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), kErrComputeDevices, "hipGetDevice");
+    check_hip(hipDeviceGetAttribute(&n_cu_,
+            hipDeviceAttributeMultiprocessorCount, dev), kErrComputeDevices,
+            "hipDeviceGetAttribute");
+    int occ = 0;
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ,
+            (const void*)k_grid_batch, kGridThreads, 0), kErrComputeDevices,
+            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    grid_blocks_ = n_cu_ * std::max(1, occ);
+    check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
+            kErrComputeDevices, "hipStreamCreateWithFlags");
+    own_stream_ = true;
+    for (auto& e : ev_)
+        check_hip(hipEventCreate(&e), kErrComputeDevices, "hipEventCreate");
+    d_bsum_ = dmalloc<Summary>(1);
+    d_ntiles_ = dmalloc<unsigned>(1);
+}
+
+Batch::~Batch()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graph();
+    free_buffers();
+    dfree(d_bsum_);
+    dfree(d_ntiles_);
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Batch::drop_graph()
+{
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+}
+
+void Batch::free_buffers()
+{
+    for (void* p : {(void*)d_desc_, (void*)d_sum_, (void*)d_combo_,
+                    (void*)d_keys_[0], (void*)d_keys_[1], (void*)d_idx_[0],
+                    (void*)d_idx_[1], d_rec_, (void*)d_meta_, d_vq_,
+                    (void*)d_tcount_, (void*)d_tstart_, (void*)d_tcursor_,
+                    (void*)d_icnt_, (void*)d_ioff_, (void*)d_part_,
+                    (void*)d_items_})
+        dfree(p);
+    if (h_sum_) (void)hipHostFree(h_sum_);
+    d_desc_ = nullptr;
+    d_sum_ = nullptr;
+    d_combo_ = nullptr;
+    d_keys_[0] = d_keys_[1] = nullptr;
+    d_idx_[0] = d_idx_[1] = nullptr;
+    d_rec_ = nullptr;
+    d_meta_ = nullptr;
+    d_vq_ = nullptr;
+    d_tcount_ = d_tstart_ = d_tcursor_ = d_icnt_ = d_ioff_ = d_part_ = nullptr;
+    d_items_ = nullptr;
+    h_sum_ = nullptr;
+    cap_points_ = cap_tiles_ = cap_combo_ = cap_jobs_ = 0;
+}
+
+int Batch::add(Plan* plan, size_t n, const float* uu, const float* vv,
+               const float* ww, const float* vis, float* grid)
+{
+    if (!plan) fail(kErrInvalidArgument, "plan is NULL");
+    if (plan->precision() != kPrecSingle)
+        fail(kErrInvalidArgument, "batched jobs need GW_SINGLE plans");
+    if (!grid || (n && (!uu || !vv || !ww || !vis)))
+        fail(kErrInvalidArgument, "device pointers must not be NULL");
+    if (plan->key_bits() + 1 > kBatchJobShift)
+        fail(kErrInvalidArgument, "grid too large for a batched job");
+    if (jobs_.size() >= 65535)
+        fail(kErrInvalidArgument, "a batch holds at most 65535 jobs");
+    jobs_.push_back(Entry{plan, n, uu, vv, ww, vis, grid});
+    dirty_ = true;
+    return (int)jobs_.size() - 1;
+}
+
+void Batch::clear()
+{
+    jobs_.clear();
+    dirty_ = true;
+}
+
+void Batch::set_stream(hipStream_t s)
+{
+    if (!s) return;
+    if (own_stream_ && stream_)
+    {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+    stream_ = s;
+    own_stream_ = false;
+    dirty_ = true;
+}
+
+void Batch::prepare()
+{
+    const size_t J = jobs_.size();
+    desc_.resize(J);
+    combo_off_.resize(J + 1);
+    size_t pts = 0, tiles = 0, combos = 0;
+    for (size_t j = 0; j < J; ++j)
+    {
+        const Entry& e = jobs_[j];
+        const Plan& pl = *e.plan;
+        BatchJob& d = desc_[j];
+        d.pinfo = pl.device_pinfo();
+        d.etab = pl.device_etab();
+        d.grid = e.grid;
+        d.G = pl.grid_size();
+        d.R = pl.region();
+        d.tile_bits = pl.tile_bits();
+        d.tile_shift = pl.tile_shift();
+        d.key_bits = pl.key_bits();
+        d.min_share = pl.min_share();
+        d.sub_h = pl.half_sub();
+        d.sub_k = pl.sub_tables();
+        d.tile_base = (unsigned)tiles;
+        d.ntiles = 1u << (2 * (pl.coord_bits() - pl.tile_bits()));
+        d.point_base = (unsigned)pts;
+        d.n = (unsigned)e.n;
+        combo_off_[j] = combos;
+        pts += e.n;
+        tiles += d.ntiles;
+        combos += pl.num_w_planes() * (size_t)pl.sub_tables() *
+                pl.sub_tables();
+        if (pts >= 0x7FFFFFFFull || tiles >= 0x7FFFFFFFull)
+            fail(kErrInvalidArgument, "batch too large (2^31 points or "
+                 "tiles)");
+    }
+    combo_off_[J] = combos;
+    n_total_ = pts;
+    tiles_total_ = tiles;
+    combo_total_ = combos;
+    const size_t pcap = std::max<size_t>(pts, 1024);
+    items_cap_ = pcap + pcap / kJobPoints + 2;
+    // Allocated on first use even for an empty batch: the scans write
+    // their partial sums whatever the counts.
+    if (!d_keys_[0] || pts > cap_points_ || tiles > cap_tiles_ ||
+            combos > cap_combo_ || J > cap_jobs_)
+    {
+        drop_graph();
+        free_buffers();
+        cap_points_ = pcap;
+        cap_tiles_ = std::max<size_t>(tiles, 1);
+        cap_combo_ = std::max<size_t>(combos, 1);
+        cap_jobs_ = std::max<size_t>(J, 1);
+        for (int i = 0; i < 2; ++i)
+        {
+            d_keys_[i] = dmalloc<unsigned long long>(cap_points_);
+            d_idx_[i] = dmalloc<unsigned>(cap_points_);
+        }
+        d_rec_ = dmalloc<RecF>(cap_points_);
+        d_meta_ = dmalloc<int4>(cap_points_);
+        d_vq_ = dmalloc<float4>(cap_points_);
+        d_tcount_ = dmalloc<unsigned>(cap_tiles_);
+        d_tstart_ = dmalloc<unsigned>(cap_tiles_);
+        d_tcursor_ = dmalloc<unsigned>(cap_tiles_);
+        d_icnt_ = dmalloc<unsigned>(cap_tiles_);
+        d_ioff_ = dmalloc<unsigned>(cap_tiles_);
+        d_part_ = dmalloc<unsigned>(
+                std::max(cap_points_, cap_tiles_) / kScanTile + 2);
+        d_items_ = dmalloc<int4>(cap_points_ + cap_points_ / kJobPoints + 2);
+        d_desc_ = dmalloc<BatchJob>(cap_jobs_);
+        d_sum_ = dmalloc<Summary>(cap_jobs_);
+        d_combo_ = dmalloc<unsigned>(cap_combo_);
+        check_hip(hipHostMalloc((void**)&h_sum_, cap_jobs_ * sizeof(Summary),
+                hipHostMallocDefault), kErrMemAlloc, "hipHostMalloc");
+    }
+    items_cap_ = cap_points_ + cap_points_ / kJobPoints + 2;
+    if (J)
+        check_hip(hipMemcpy(d_desc_, desc_.data(), J * sizeof(BatchJob),
+                hipMemcpyHostToDevice), kErrMemCopy, "upload job table");
+    const unsigned nt = (unsigned)tiles;
+    check_hip(hipMemcpy(d_ntiles_, &nt, sizeof(unsigned),
+            hipMemcpyHostToDevice), kErrMemCopy, "upload tile count");
+}
+
+void Batch::enqueue()
+{
+    const hipStream_t s = stream_;
+    const size_t J = jobs_.size();
+    for (size_t j = 0; j < J; ++j)
+    {
+        const Entry& e = jobs_[j];
+        const Plan& pl = *e.plan;
+        const BatchJob& d = desc_[j];
+        const int P = (int)pl.num_w_planes();
+        const int K = pl.sub_tables();
+        const int n_combo = P * K * K;
+        unsigned* combo = d_combo_ + combo_off_[j];
+        hipLaunchKernelGGL(k_init, dim3(std::max(1, std::min(1024,
+                (n_combo + 255) / 256))), dim3(256), 0, s, d_sum_ + j, combo,
+                n_combo);
+        if (e.n)
+        {
+            const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
+                    (e.n + 255) / 256, (size_t)n_cu_ * 4));
+            hipLaunchKernelGGL((k_locate<float, false, float, float, RecF>),
+                    dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
+                    e.vis, pl.grid_scale_f(), pl.w_scale_f(), pl.oversample(),
+                    d.G, P, pl.device_support(), d.sub_h, K, d.tile_bits,
+                    pl.combo_bits(), d.key_bits, d_keys_[0] + d.point_base,
+                    d_idx_[0] + d.point_base,
+                    static_cast<RecF*>(d_rec_) + d.point_base, combo,
+                    d_sum_ + j, (unsigned*)nullptr,
+                    (unsigned long long*)nullptr,
+                    (unsigned long long*)nullptr, 0,
+                    (unsigned long long)j << kBatchJobShift, d.point_base);
+        }
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, P, K * K,
+                (const unsigned*)combo, pl.device_pinfo(),
+                pl.device_normtab(), d_sum_ + j);
+    }
+    check_hip(hipGetLastError(), kErrKernelLaunch, "batch index launch");
+    fill_zero_u32(d_tcount_, tiles_total_, s);
+    fill_zero_u32(d_tcursor_, tiles_total_, s);
+    const int pblocks = (int)std::max<size_t>(1, std::min<size_t>(
+            (n_total_ + 255) / 256, (size_t)n_cu_ * 8));
+    const int tblocks = (int)std::max<size_t>(1, std::min<size_t>(
+            (tiles_total_ + 255) / 256, (size_t)n_cu_ * 8));
+    hipLaunchKernelGGL(k_tile_count, dim3(pblocks), dim3(256), 0, s,
+            n_total_, (const unsigned long long*)d_keys_[0],
+            (const BatchJob*)d_desc_, d_tcount_);
+    scan_u32_enqueue(d_tcount_, d_tstart_, d_ntiles_, tiles_total_,
+            &d_bsum_->nv32, d_part_, s);
+    hipLaunchKernelGGL(k_tile_scatter, dim3(pblocks), dim3(256), 0, s,
+            n_total_, (const unsigned long long*)d_keys_[0],
+            (const unsigned*)d_idx_[0], (const BatchJob*)d_desc_,
+            (const unsigned*)d_tstart_, d_tcursor_, d_keys_[1], d_idx_[1]);
+    hipLaunchKernelGGL(k_tile_items, dim3(tblocks), dim3(256), 0, s,
+            (const unsigned*)d_ntiles_, (const unsigned*)d_tcount_, d_icnt_);
+    scan_u32_enqueue(d_icnt_, d_ioff_, d_ntiles_, tiles_total_,
+            &d_bsum_->total_items, d_part_, s);
+    hipLaunchKernelGGL(k_write_tile_items, dim3(tblocks), dim3(256), 0, s,
+            (unsigned)tiles_total_, (const unsigned*)d_tcount_,
+            (const unsigned*)d_tstart_, (const unsigned*)d_ioff_,
+            (const BatchJob*)d_desc_, (int)J, (unsigned)items_cap_, d_items_);
+    hipLaunchKernelGGL(k_item_sort, dim3(grid_blocks_), dim3(1024), 0, s,
+            (const Summary*)d_bsum_, (const int4*)d_items_, d_keys_[1],
+            d_idx_[1]);
+    hipLaunchKernelGGL(k_gather_batch, dim3(pblocks), dim3(256), 0, s,
+            (const Summary*)d_bsum_, (const unsigned long long*)d_keys_[1],
+            (const unsigned*)d_idx_[1], (const RecF*)d_rec_,
+            (const BatchJob*)d_desc_, d_meta_, (float4*)d_vq_);
+    hipLaunchKernelGGL(k_grid_batch, dim3(grid_blocks_), dim3(kGridThreads),
+            0, s, (const Summary*)d_bsum_, (const int4*)d_items_,
+            (const BatchJob*)d_desc_, (const int4*)d_meta_,
+            (const float4*)d_vq_);
+    check_hip(hipGetLastError(), kErrKernelLaunch, "batch launch");
+}
+
+void Batch::run()
+{
+    prepare();
+    dirty_ = false;
+    drop_graph();
+    check_hip(hipEventRecord(ev_[0], stream_), kErrKernelLaunch, "event");
+    enqueue();
+    check_hip(hipEventRecord(ev_[1], stream_), kErrKernelLaunch, "event");
+    timed_ = true;
+}
+
+void Batch::capture()
+{
+    prepare();
+    drop_graph();
+    check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+            kErrKernelLaunch, "hipStreamBeginCapture");
+    try
+    {
+        enqueue();
+    }
+    catch (...)
+    {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    check_hip(hipStreamEndCapture(stream_, &graph_), kErrKernelLaunch,
+            "hipStreamEndCapture");
+    check_hip(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0),
+            kErrKernelLaunch, "hipGraphInstantiate");
+    dirty_ = false;
+}
+
+void Batch::launch()
+{
+    if (dirty_ || !exec_) capture();
+    check_hip(hipEventRecord(ev_[0], stream_), kErrKernelLaunch, "event");
+    check_hip(hipGraphLaunch(exec_, stream_), kErrKernelLaunch,
+            "hipGraphLaunch");
+    check_hip(hipEventRecord(ev_[1], stream_), kErrKernelLaunch, "event");
+    timed_ = true;
+}
+
+void Batch::results(int job, size_t* skipped, double* norm_delta,
+                    unsigned long long* updates)
+{
+    if (job < 0 || (size_t)job >= jobs_.size())
+        fail(kErrInvalidArgument, "job index out of range");
+    check_hip(hipMemcpyAsync(h_sum_, d_sum_, jobs_.size() * sizeof(Summary),
+            hipMemcpyDeviceToHost, stream_), kErrMemCopy, "read summaries");
+    check_hip(hipStreamSynchronize(stream_), kErrKernelLaunch,
+            "stream synchronize");
+    const Summary& sm = h_sum_[job];
+    if (skipped) *skipped = (size_t)sm.skipped;
+    if (norm_delta) *norm_delta = sm.norm_delta;
+    if (updates) *updates = sm.updates;
+}
+
+double Batch::last_ms()
+{
+    if (!timed_) return 0.0;
+    check_hip(hipEventSynchronize(ev_[1]), kErrKernelLaunch, "event sync");
+    float ms = 0.0f;
+    check_hip(hipEventElapsedTime(&ms, ev_[0], ev_[1]), kErrKernelLaunch,
+            "elapsed");
+    return ms;
 }
 
 // Zero-fill helpers used by the C ABI layer (a fill kernel rather than

@@ -213,6 +213,38 @@ public:
     size_t num_w_planes() const { return P_; }
     int max_support() const { return s_max_; }
     int timings(double* ms, int max_n);
+    // Scatter shape: tile side 2^tile_bits (-1: the largest whose LDS region
+    // fits), at most slot_cap footprint cells per lane (0: the largest the
+    // precision allows), at least min_share points per stream chunk (0: the
+    // default).  Waits for the plan's stream before changing its tables.
+    void configure(int tile_bits, int slot_cap, int min_share);
+    int tile_bits() const { return tile_bits_; }
+    int slot_cap() const { return slot_cap_; }
+    int min_share() const { return min_share_; }
+    int region() const { return region_; }
+    int max_tile_bits() const;
+    // Autotuner: runs locate + scatter on the given device points (grid
+    // into scratch_grid, which is overwritten) for scatter shapes around
+    // the current one -- tile side, then slots per lane, then chunk minimum,
+    // each kept if faster (best of reps timed runs) -- and keeps the
+    // fastest.  Returns its device milliseconds per call.
+    double autotune(size_t n, const void* uu, const void* vv, const void* ww,
+                    const void* vis, void* scratch_grid, int reps);
+    // Device tables and geometry for batched launches.
+    const PlaneInfo* device_pinfo() const { return d_pinfo_; }
+    const void* device_etab() const { return d_etab_; }
+    const double* device_normtab() const { return d_normtab_; }
+    const int* device_support() const { return d_support_; }
+    int coord_bits() const { return coord_bits_; }
+    int combo_bits() const { return combo_bits_; }
+    int tile_shift() const { return tile_shift_; }
+    int key_bits() const { return key_bits_; }
+    int oversample() const { return ov_; }
+    int half_sub() const { return h_; }
+    int sub_tables() const { return K_; }
+    float grid_scale_f() const { return gscale_f_; }
+    float w_scale_f() const { return ws_f_; }
+    int grid_blocks() const { return grid_blocks_; }
 
 private:
     void ensure_capacity(size_t n);
@@ -243,6 +275,7 @@ private:
     std::vector<int> support_;
     std::vector<PlaneInfo> pinfo_;
     int min_share_ = kMinShare; // least points per scatter stream
+    int slot_cap_ = 8;          // most footprint cells per lane
     int device_ = 0;
     int n_cu_ = 0;
     int grid_blocks_ = 0;
@@ -277,6 +310,101 @@ private:
     Summary* h_summary_ = nullptr;       // pinned
     hipEvent_t ev_[5] = {};
     bool timed_ = false;
+};
+
+// One gridding job of a batch, as the batch kernels read it.
+struct BatchJob
+{
+    const PlaneInfo* pinfo;   // the job's plan tables
+    const void* etab;
+    float* grid;              // 2*G*G floats
+    int G, R, tile_bits, tile_shift;
+    int key_bits, min_share, sub_h, sub_k;
+    unsigned tile_base;       // first global tile of the job
+    unsigned ntiles;          // 4^(coord_bits - tile_bits)
+    unsigned point_base;      // first global point of the job
+    unsigned n;
+};
+
+// Many independent FP32 gridding jobs (each with its own plan: kernel set,
+// grid geometry and scatter shape; its own device points and grid) run as
+// one pipeline: per-job index stages, then a counting sort of all points by
+// (job, tile), work items sorted in LDS, and ONE persistent scatter launch
+// over the work items of every job.  The enqueue sequence uses only the
+// library's own kernels (no memset, no library sort), so it can be captured
+// once into a HIP graph and replayed.
+class Batch
+{
+public:
+    Batch();
+    ~Batch();
+    // Adds a job; returns its index.  plan must be GW_SINGLE and outlive
+    // the batch's use of it; pointers are device memory.
+    int add(Plan* plan, size_t n, const float* uu, const float* vv,
+            const float* ww, const float* vis, float* grid);
+    void clear();
+    size_t size() const { return jobs_.size(); }
+    void set_stream(hipStream_t s);
+    hipStream_t stream() const { return stream_; }
+    // Enqueue the whole pipeline on the stream (eager launches).
+    void run();
+    // Capture the pipeline of the current jobs into a graph (instantiated).
+    void capture();
+    // Replay the captured graph (captures first if the jobs changed).
+    void launch();
+    // Waits for the stream; the job's skip count, norm increment and
+    // update count of the last run / launch.
+    void results(int job, size_t* skipped, double* norm_delta,
+                 unsigned long long* updates);
+    // Device milliseconds of the last run / launch (waits).
+    double last_ms();
+
+private:
+    struct Entry
+    {
+        Plan* plan;
+        size_t n;
+        const float *uu, *vv, *ww, *vis;
+        float* grid;
+    };
+    void prepare();
+    void enqueue();
+    void free_buffers();
+    void drop_graph();
+
+    std::vector<Entry> jobs_;
+    std::vector<BatchJob> desc_;
+    std::vector<size_t> combo_off_;
+    bool dirty_ = true;
+    size_t n_total_ = 0, tiles_total_ = 0, combo_total_ = 0;
+    size_t items_cap_ = 0;
+    size_t cap_points_ = 0, cap_tiles_ = 0, cap_combo_ = 0, cap_jobs_ = 0;
+    BatchJob* d_desc_ = nullptr;
+    Summary* d_sum_ = nullptr;        // per job
+    Summary* d_bsum_ = nullptr;       // batch totals (nv32, total_items)
+    Summary* h_sum_ = nullptr;        // pinned copy of d_sum_
+    unsigned* d_combo_ = nullptr;
+    unsigned long long* d_keys_[2] = {nullptr, nullptr};
+    unsigned* d_idx_[2] = {nullptr, nullptr};
+    void* d_rec_ = nullptr;
+    int4* d_meta_ = nullptr;
+    void* d_vq_ = nullptr;
+    unsigned* d_tcount_ = nullptr;
+    unsigned* d_tstart_ = nullptr;
+    unsigned* d_tcursor_ = nullptr;
+    unsigned* d_icnt_ = nullptr;
+    unsigned* d_ioff_ = nullptr;
+    unsigned* d_part_ = nullptr;
+    unsigned* d_ntiles_ = nullptr;
+    int4* d_items_ = nullptr;
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    hipEvent_t ev_[2] = {};
+    bool timed_ = false;
+    int grid_blocks_ = 0;
+    int n_cu_ = 0;
 };
 
 // Zero-fill with a kernel (capture-safe, unlike hipMemsetAsync).

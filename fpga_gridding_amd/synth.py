@@ -373,3 +373,70 @@ def workload_arrays(wl):
                 vis=wl.vis, weight=wl.weight,
                 cell_size_rad=np.float32(wl.cell_size_rad),
                 w_scale=np.float32(wl.w_scale), grid_size=wl.grid_size)
+
+
+@dataclass
+class KernelSet:
+    """A W-kernel set shared by several jobs (planes, support table, cube)."""
+    num_w_planes: int
+    support: np.ndarray
+    oversample: int
+    conv_size_half: int
+    conv_func: np.ndarray
+    w_scale: float
+
+
+def batch_mix(n_jobs=64, seed=7, n_range=(10_000, 5_000_000),
+              grids=(1024, 2048, 4096, 8192), n_kernel_sets=8,
+              planes=(16, 128), s_max_range=(4, 36), ov=4, base=None,
+              fov_deg_at_8192=5.0):
+    """Independent gridding jobs of uneven size for batched launches.
+
+    Each job takes a random subset (log-uniform size in n_range, kept in the
+    base order) of the base workload's uvw and visibilities (default: the
+    512-station workload), a grid side from ``grids`` and one of
+    ``n_kernel_sets`` kernel sets (planes uniform in ``planes``, support
+    s_min = min(4, s_max) .. s_max with s_max uniform in ``s_max_range``,
+    kernels as in wkernel_cube, w_scale = (P-1)^2 / w_max).  The field of view
+    is fov_deg_at_8192 * 8192 / G, so that every grid side covers the same
+    fraction of the uv plane.  Returns (jobs: list of Workload, kernel set
+    index per job, kernel sets)."""
+    if base is None:
+        base = full_scale()
+    rng = np.random.default_rng(seed)
+    w_max = float(np.max(np.abs(base.ww.astype(np.float64))))
+    sets = []
+    for _ in range(n_kernel_sets):
+        P = int(rng.integers(planes[0], planes[1] + 1))
+        s_max = int(rng.integers(s_max_range[0], s_max_range[1] + 1))
+        s_min = min(4, s_max)
+        support = support_table(P, s_min, s_max)
+        c = cube_half_size(ov, s_max)
+        cube = wkernel_cube(support, ov, c, w_max)
+        sets.append(KernelSet(num_w_planes=P, support=support, oversample=ov,
+                              conv_size_half=c,
+                              conv_func=cube_to_conv_func(cube),
+                              w_scale=float(np.float32((P - 1) ** 2 / w_max))))
+    vis2 = base.vis.reshape(-1, 2)
+    lo, hi = np.log(n_range[0]), np.log(n_range[1])
+    jobs, which = [], []
+    for j in range(n_jobs):
+        k = int(rng.integers(0, n_kernel_sets))
+        G = int(grids[int(rng.integers(0, len(grids)))])
+        n = int(min(base.num_points, round(np.exp(rng.uniform(lo, hi)))))
+        idx = np.sort(rng.choice(base.num_points, n, replace=False))
+        ks = sets[k]
+        fov = fov_deg_at_8192 * 8192.0 / G
+        cell = float(np.float32(math.radians(fov) / G))
+        jobs.append(Workload(
+            name=f"job{j}", num_w_planes=ks.num_w_planes, support=ks.support,
+            oversample=ov, conv_size_half=ks.conv_size_half,
+            conv_func=ks.conv_func, uu=np.ascontiguousarray(base.uu[idx]),
+            vv=np.ascontiguousarray(base.vv[idx]),
+            ww=np.ascontiguousarray(base.ww[idx]),
+            vis=np.ascontiguousarray(vis2[idx]).reshape(-1),
+            weight=np.ones(n, np.float32), cell_size_rad=cell,
+            w_scale=ks.w_scale, grid_size=G,
+            params=dict(kernel_set=k, fov_deg=fov, base=base.name)))
+        which.append(k)
+    return jobs, which, sets

@@ -136,6 +136,30 @@ GW_EXPORT
 void gw_plan_results(gw_plan* plan, size_t* num_skipped, double* norm_delta,
         unsigned long long* num_updates, int* status);
 
+/* Scatter shape of a plan (no counterpart in the reference; a tuning knob,
+ * results do not depend on it): tile side 2^tile_log2 of footprint origins
+ * per work item (-1: the largest whose FP64 LDS region fits), at most
+ * slot_cap footprint cells per lane (0: the largest: 8 for FP32-product
+ * plans, 6 for GW_DOUBLE), at least min_share points per stream chunk
+ * (0: 8).  Waits for the plan's stream. */
+GW_EXPORT
+void gw_plan_set_tuning(gw_plan* plan, int tile_log2, int slot_cap,
+        int min_share, int* status);
+
+/* values[0..4] = tile_log2, slot_cap, min_share, LDS region side (0: none),
+ * largest tile_log2 whose region fits.  Returns the number written. */
+GW_EXPORT int gw_plan_get_tuning(gw_plan* plan, int* values, int max_n);
+
+/* Autotuner: grids the given device points (into d_scratch_grid, sized as
+ * for gw_grid_device and overwritten) with scatter shapes around the
+ * current one -- tile side, then slots per lane, then chunk minimum, each
+ * kept when faster over `reps` timed calls -- and keeps the fastest on the
+ * plan.  *best_ms (may be NULL) gets its device milliseconds per call. */
+GW_EXPORT
+void gw_plan_autotune(gw_plan* plan, size_t num_points, const void* d_uu,
+        const void* d_vv, const void* d_ww, const void* d_vis,
+        void* d_scratch_grid, int reps, double* best_ms, int* status);
+
 /* Device milliseconds of the stages of the last gw_grid_device call,
  * measured with HIP events on the plan's stream:
  *   ms[0] locate (index math, skip count, histograms, sort keys)
@@ -147,8 +171,46 @@ void gw_plan_results(gw_plan* plan, size_t* num_skipped, double* norm_delta,
 GW_EXPORT int gw_plan_timings(gw_plan* plan, double* ms, int max_n);
 
 /* ---------------------------------------------------------------------- */
+/* Batches: many FP32 jobs in one pipeline and one persistent scatter      */
+/* launch (no counterpart in the reference; it grids one job per call)     */
+/* ---------------------------------------------------------------------- */
+
+typedef struct gw_batch gw_batch;
+
+/* A batch on the current HIP device, with its own non-blocking stream. */
+GW_EXPORT gw_batch* gw_batch_create(int* status);
+GW_EXPORT void gw_batch_free(gw_batch* batch);
+
+/* Add a job: a GW_SINGLE plan (kernel set, grid geometry, scatter shape;
+ * it may serve several jobs and must outlive the batch's use of it) and
+ * device arrays as for gw_grid_device.  Returns the job index (-1 on
+ * error).  Jobs must not share a grid. */
+GW_EXPORT int gw_batch_add(gw_batch* batch, gw_plan* plan, size_t num_points,
+        const float* d_uu, const float* d_vv, const float* d_ww,
+        const float* d_vis, float* d_grid, int* status);
+GW_EXPORT void gw_batch_clear(gw_batch* batch);
+GW_EXPORT void gw_batch_set_stream(gw_batch* batch, void* hip_stream);
+
+/* Enqueue the pipeline of all jobs with ordinary launches. */
+GW_EXPORT void gw_batch_run(gw_batch* batch, int* status);
+/* Capture the pipeline of the current jobs into a HIP graph: only the
+ * library's own kernels, no memset nodes, no library sort. */
+GW_EXPORT void gw_batch_capture(gw_batch* batch, int* status);
+/* Replay the captured graph (captures first if the job list changed). */
+GW_EXPORT void gw_batch_launch(gw_batch* batch, int* status);
+/* Waits; num_skipped, norm increment and update count of one job of the
+ * last run / launch. */
+GW_EXPORT void gw_batch_results(gw_batch* batch, int job, size_t* num_skipped,
+        double* norm_delta, unsigned long long* num_updates, int* status);
+/* Device milliseconds of the last run / launch (waits). */
+GW_EXPORT double gw_batch_last_ms(gw_batch* batch);
+
+/* ---------------------------------------------------------------------- */
 /* Device memory helpers (current HIP device)                              */
 /* ---------------------------------------------------------------------- */
+/* The copy and memset helpers first wait for all work on the device (any
+ * stream, the plans' and batches' included) and complete before they
+ * return, so they order correctly with asynchronous gridding calls. */
 
 GW_EXPORT int   gw_device_count(void);
 GW_EXPORT void* gw_device_malloc(size_t bytes, int* status);
