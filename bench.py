@@ -185,6 +185,258 @@ def precision_variants(wl, d_uvw, args):
     return out
 
 
+MIX_WORKLOAD = ("64 jobs from the 512-station workload: random subsets of "
+                "log-uniform size 10k-5M points, grid from {1024, 2048, 4096, "
+                "8192}^2 (5 deg FOV at 8192), 8 kernel sets of 16-128 "
+                "w-planes, support 4..4-36, oversample 4 (seed 7)")
+
+
+def _mix_setup(autotune=True):
+    """The 64-job mix on the device: plans (one per kernel set and grid
+    side, autotuned on their largest job), points and one grid per job."""
+    import fpga_gridding_amd as fg
+    from fpga_gridding_amd import synth
+    base = synth.full_scale()
+    jobs, _, sets = synth.batch_mix(64, seed=7, base=base)
+    plans = {}
+    for wl in jobs:
+        key = (wl.params["kernel_set"], wl.grid_size)
+        if key not in plans:
+            plans[key] = fg.Plan.for_workload(wl)
+    dev = []
+    for wl in jobs:
+        d = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww,
+                                                    wl.vis)]
+        g = fg.DeviceArray(8 * wl.grid_size ** 2)
+        g.zero()
+        dev.append((d, g))
+    tune_s = 0.0
+    if autotune:
+        t0 = time.perf_counter()
+        largest = {}
+        for j, wl in enumerate(jobs):
+            key = (wl.params["kernel_set"], wl.grid_size)
+            if key not in largest or wl.num_points > \
+                    jobs[largest[key]].num_points:
+                largest[key] = j
+        for key, j in largest.items():
+            d, g = dev[j]
+            plans[key].autotune(jobs[j].num_points, *d, g, reps=1)
+            g.zero()
+        tune_s = time.perf_counter() - t0
+    return jobs, sets, plans, dev, tune_s
+
+
+def bench_batched(args):
+    """Batched jobs: the 64-job mix in one pipeline with one persistent
+    scatter launch, replayed from a HIP graph, against 64 sequential plain
+    per-job calls (gw_grid_device + gw_plan_results each) on the same plans
+    and resident data."""
+    import fpga_gridding_amd as fg
+    jobs, sets, plans, dev, tune_s = _mix_setup()
+    key = [(wl.params["kernel_set"], wl.grid_size) for wl in jobs]
+    n_total = sum(wl.num_points for wl in jobs)
+
+    def sequential():
+        out = []
+        for j, wl in enumerate(jobs):
+            d, g = dev[j]
+            plans[key[j]].grid(wl.num_points, *d, g)
+            out.append(plans[key[j]].results())
+        return out
+
+    b = fg.Batch()
+    for j, wl in enumerate(jobs):
+        d, g = dev[j]
+        b.add(plans[key[j]], wl.num_points, *d, g)
+    b.capture()
+
+    # One pass each from zeroed grids: per-job results must agree.
+    small = sorted(range(len(jobs)), key=lambda j: jobs[j].grid_size)[:4]
+    for _, g in dev:
+        g.zero()
+    ref = sequential()
+    ref_g = {j: dev[j][1].download(np.empty(2 * jobs[j].grid_size ** 2,
+                                            np.float32)) for j in small}
+    for _, g in dev:
+        g.zero()
+    b.launch()
+    agree = True
+    worst = 0.0
+    for j in range(len(jobs)):
+        sk, nd, nu = b.results(j)
+        agree &= (sk, nu) == (ref[j][0], ref[j][2])
+        agree &= abs(nd - ref[j][1]) <= 1e-12 * max(abs(ref[j][1]), 1.0)
+    for j in small:
+        gj = dev[j][1].download(np.empty(2 * jobs[j].grid_size ** 2,
+                                         np.float32))
+        s = np.abs(ref_g[j]).max()
+        if s > 0:
+            worst = max(worst, float(np.abs(gj.astype(np.float64) -
+                                            ref_g[j]).max() / s))
+
+    for _ in range(args.warmup):
+        sequential()
+    fg.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sequential()
+    fg.device_synchronize()
+    seq_ms = (time.perf_counter() - t0) / args.steps * 1e3
+
+    for _ in range(args.warmup):
+        b.launch()
+        b.results(0)
+    fg.device_synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.launch()
+        b.results(0)
+        dev_ms.append(b.last_ms())
+    fg.device_synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    updates = sum(r[2] for r in ref)
+    out = {
+        "metric": "Mvis/s, batched jobs (one hipGraph launch) vs sequential "
+                  "plain launches",
+        "value": n_total / (ms * 1e-3) / 1e6, "unit": "Mvis/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": MIX_WORKLOAD, "num_jobs": len(jobs),
+                   "num_points": n_total, "plans": len(plans),
+                   "inputs": "resident in HBM"},
+        "graph_device_ms": float(np.mean(dev_ms)),
+        "grid_updates_per_s": updates / (ms * 1e-3),
+        "sequential": {"value": n_total / (seq_ms * 1e-3) / 1e6,
+                       "unit": "Mvis/s", "ms_per_step": seq_ms},
+        "speedup_vs_sequential": seq_ms / ms,
+        "autotune_s": tune_s,
+        "check": {"results_identical": bool(agree),
+                  "grid_max_rel_4_smallest": worst},
+    }
+    b.close()
+    return out
+
+
+def bench_served(args):
+    """Served: requests drawn from the 64-job mix arrive as a Poisson stream;
+    each adds into a resident grid of its kernel set and grid side.  The
+    server (gw_server_*) is compared with the plain per-job loop (upload,
+    gw_grid_device, gw_plan_results, one request at a time) on the same
+    arrivals."""
+    import threading
+    import fpga_gridding_amd as fg
+    jobs, sets, plans, dev, tune_s = _mix_setup(autotune=False)
+    rng = np.random.default_rng(11)
+    R = args.requests
+    order = rng.integers(0, len(jobs), R)
+    arrive = np.cumsum(rng.exponential(1.0 / args.rate, R))
+    n_req = [jobs[i].num_points for i in order]
+    n_total = int(sum(n_req))
+
+    # Plain per-job loop on the plans above (kernel tables resident, a
+    # device grid per job), one staging buffer for the uploaded points.
+    n_max = max(wl.num_points for wl in jobs)
+    stage = [fg.DeviceArray(4 * n_max) for _ in range(3)] + \
+        [fg.DeviceArray(8 * n_max)]
+    key = [(wl.params["kernel_set"], wl.grid_size) for wl in jobs]
+    lat_plain = []
+    t0 = time.perf_counter()
+    for r in range(R):
+        wait = t0 + arrive[r] - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        wl = jobs[order[r]]
+        for d, a in zip(stage, (wl.uu, wl.vv, wl.ww, wl.vis)):
+            d.upload(a)
+        pl = plans[key[order[r]]]
+        pl.grid(wl.num_points, *stage, dev[order[r]][1])
+        pl.results()
+        lat_plain.append(time.perf_counter() - (t0 + arrive[r]))
+    plain_span = time.perf_counter() - t0
+    for x in stage:
+        x.close()
+    for d, g in dev:
+        for x in (*d, g):
+            x.close()
+    for p in plans.values():
+        p.close()
+
+    sv = fg.Server(args.hbm_budget_gib << 30, max_batch_jobs=16,
+                   num_streams=2)
+    kid = [sv.add_kernel_set(k.num_w_planes, k.support, k.oversample,
+                             k.conv_size_half, k.conv_func) for k in sets]
+    gids = {}
+    for wl in jobs:
+        k = (wl.params["kernel_set"], wl.grid_size)
+        if k not in gids:
+            gids[k] = sv.create_grid(wl.grid_size)
+
+    def submit(i):
+        wl = jobs[i]
+        return sv.submit(kid[wl.params["kernel_set"]], wl.grid_size,
+                         wl.cell_size_rad, wl.w_scale, wl.uu, wl.vv, wl.ww,
+                         wl.vis, grid_id=gids[key[i]])
+
+    # Warm-up: every job once (plans built and resident).
+    for rid in [submit(i) for i in range(len(jobs))]:
+        sv.wait(rid)
+    rids = [None] * R
+
+    def producer():
+        base = time.perf_counter()
+        for r in range(R):
+            wait = base + arrive[r] - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            rids[r] = submit(order[r])
+
+    th = threading.Thread(target=producer)
+    t0 = time.perf_counter()
+    th.start()
+    res = []
+    for r in range(R):
+        while rids[r] is None:
+            time.sleep(1e-4)
+        res.append(sv.wait(rids[r])[0])
+    th.join()
+    served_span = time.perf_counter() - t0
+    lat = np.array([x.t_done - x.t_submit for x in res])
+    first = min(x.t_submit for x in res)
+    last = max(x.t_done for x in res)
+    st = sv.stats()
+    sv.close()
+    lp = np.array(lat_plain)
+    return {
+        "metric": "sustained Mvis/s and p50/p99 job latency, served "
+                  "requests vs the plain per-job loop",
+        "value": n_total / (last - first) / 1e6, "unit": "Mvis/s",
+        "n_gpus": 1, "steps": R, "warmup": len(jobs),
+        "ms_per_step": (last - first) / R * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": MIX_WORKLOAD, "requests": R,
+                   "arrivals": f"Poisson, {args.rate} requests/s",
+                   "num_points": n_total, "grids": "resident per kernel "
+                   "set and grid side", "hbm_budget_gib":
+                   args.hbm_budget_gib, "max_batch_jobs": 16,
+                   "streams": 2},
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "offered_mvis_s": n_total / arrive[-1] / 1e6,
+        "served_span_s": served_span,
+        "server_stats": st,
+        "plain_loop": {"value": n_total / plain_span / 1e6,
+                       "unit": "Mvis/s",
+                       "latency_ms": {"p50": float(np.percentile(lp, 50)
+                                                   * 1e3),
+                                      "p99": float(np.percentile(lp, 99)
+                                                   * 1e3)}},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +446,17 @@ def main():
                     help="skip the CPU baseline and the oracle error sample")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP16 / BF16 plan timings")
+    ap.add_argument("--config", choices=("plain", "batched", "served"),
+                    default="plain",
+                    help="plain: the 512-station workload (default); "
+                         "batched: the 64-job mix in one graph launch; "
+                         "served: Poisson requests through the server")
+    ap.add_argument("--rate", type=float, default=200.0,
+                    help="served: request arrival rate (requests/s)")
+    ap.add_argument("--requests", type=int, default=400,
+                    help="served: number of requests")
+    ap.add_argument("--hbm-budget-gib", type=int, default=64,
+                    help="served: device-memory budget of the server")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiler calibration: copy this many bytes once "
                          "with the library's k_copy kernel and exit")
@@ -213,6 +476,12 @@ def main():
         src.zero()
         dst.copy_from(src)
         print(json.dumps({"copy_bytes": args.copy_bytes, "kernel": "k_copy"}))
+        return
+    if args.config == "batched":
+        print(json.dumps(bench_batched(args)))
+        return
+    if args.config == "served":
+        print(json.dumps(bench_served(args)))
         return
 
     wl = synth.config(1)

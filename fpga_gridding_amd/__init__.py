@@ -41,6 +41,9 @@ EXPORTED_SYMBOLS = (
     "gw_batch_create", "gw_batch_free", "gw_batch_add", "gw_batch_clear",
     "gw_batch_set_stream", "gw_batch_run", "gw_batch_capture",
     "gw_batch_launch", "gw_batch_results", "gw_batch_last_ms",
+    "gw_server_create", "gw_server_free", "gw_server_add_kernel_set",
+    "gw_server_create_grid", "gw_server_read_grid", "gw_server_free_grid",
+    "gw_server_submit", "gw_server_wait", "gw_server_get_stats",
 )
 
 
@@ -64,6 +67,22 @@ _szp = ct.POINTER(ct.c_size_t)
 _dp = ct.POINTER(ct.c_double)
 _ip = ct.POINTER(ct.c_int)
 _vp = ct.c_void_p
+
+
+class RequestResult(ct.Structure):
+    """gw_request_result."""
+    _fields_ = [("status", ct.c_int), ("num_skipped", ct.c_size_t),
+                ("norm_delta", ct.c_double),
+                ("num_updates", ct.c_ulonglong),
+                ("t_submit", ct.c_double), ("t_admit", ct.c_double),
+                ("t_start", ct.c_double), ("t_done", ct.c_double)]
+
+
+class ServerStats(ct.Structure):
+    """gw_server_stats."""
+    _fields_ = [(name, ct.c_size_t) for name in (
+        "completed", "batches", "max_batch", "plans_evicted", "points",
+        "device_bytes", "resident_plans", "pending")]
 
 
 def _wproj_args(fp, real):
@@ -156,6 +175,28 @@ def lib():
     L.gw_batch_results.restype = None
     L.gw_batch_last_ms.argtypes = [_vp]
     L.gw_batch_last_ms.restype = ct.c_double
+    L.gw_server_create.argtypes = [_sz, ct.c_int, ct.c_int, _ip]
+    L.gw_server_create.restype = _vp
+    L.gw_server_free.argtypes = [_vp]
+    L.gw_server_free.restype = None
+    L.gw_server_add_kernel_set.argtypes = [_vp, _sz, _i32p, ct.c_int,
+                                           ct.c_int, _f32p, _ip]
+    L.gw_server_add_kernel_set.restype = ct.c_int
+    L.gw_server_create_grid.argtypes = [_vp, ct.c_int, _ip]
+    L.gw_server_create_grid.restype = ct.c_int
+    L.gw_server_read_grid.argtypes = [_vp, ct.c_int, _f32p, _ip]
+    L.gw_server_read_grid.restype = None
+    L.gw_server_free_grid.argtypes = [_vp, ct.c_int]
+    L.gw_server_free_grid.restype = None
+    L.gw_server_submit.argtypes = [_vp, ct.c_int, ct.c_int, ct.c_double,
+                                   ct.c_double, _sz, _vp, _vp, _vp, _vp,
+                                   ct.c_int, _vp, _ip]
+    L.gw_server_submit.restype = ct.c_longlong
+    L.gw_server_wait.argtypes = [_vp, ct.c_longlong, ct.c_double,
+                                 ct.POINTER(RequestResult), _ip]
+    L.gw_server_wait.restype = ct.c_int
+    L.gw_server_get_stats.argtypes = [_vp, ct.POINTER(ServerStats)]
+    L.gw_server_get_stats.restype = None
     _lib = L
     return L
 
@@ -549,6 +590,96 @@ class Batch:
             lib().gw_batch_free(self.ptr)
             self.ptr = None
             self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop on the current device (gw_server_*): requests are the
+    arguments of oskar_grid_wproj_f on host arrays; admitted by the device
+    memory budget and gridded in batches on several streams."""
+
+    def __init__(self, hbm_budget, max_batch_jobs=64, num_streams=2):
+        st = ct.c_int(0)
+        self.ptr = lib().gw_server_create(int(hbm_budget), max_batch_jobs,
+                                          num_streams, ct.byref(st))
+        _check(st, "gw_server_create")
+        self._keep = {}
+
+    def add_kernel_set(self, num_w_planes, support, oversample,
+                       conv_size_half, conv_func):
+        st = ct.c_int(0)
+        k = lib().gw_server_add_kernel_set(
+            self.ptr, num_w_planes, _c(support, np.int32), oversample,
+            conv_size_half, _c(conv_func, np.float32), ct.byref(st))
+        _check(st, "gw_server_add_kernel_set")
+        return k
+
+    def create_grid(self, grid_size):
+        st = ct.c_int(0)
+        g = lib().gw_server_create_grid(self.ptr, grid_size, ct.byref(st))
+        _check(st, "gw_server_create_grid")
+        return g
+
+    def read_grid(self, grid_id, grid_size):
+        out = np.empty(2 * grid_size * grid_size, np.float32)
+        st = ct.c_int(0)
+        lib().gw_server_read_grid(self.ptr, grid_id, out, ct.byref(st))
+        _check(st, "gw_server_read_grid")
+        return out
+
+    def free_grid(self, grid_id):
+        lib().gw_server_free_grid(self.ptr, grid_id)
+
+    def submit(self, kernel_set, grid_size, cell_size_rad, w_scale, uu, vv,
+               ww, vis, grid_id=-1):
+        """Queues a request; returns its id.  Without a resident grid the
+        result grid is returned by wait()."""
+        arrs = [_c(uu, np.float32), _c(vv, np.float32), _c(ww, np.float32),
+                _c(vis, np.float32)]
+        n = arrs[0].size
+        if not (arrs[1].size == arrs[2].size == n and arrs[3].size == 2 * n):
+            raise ValueError("uu, vv, ww need n values and vis 2n")
+        out = None
+        if grid_id < 0:
+            out = np.empty(2 * grid_size * grid_size, np.float32)
+        st = ct.c_int(0)
+        rid = lib().gw_server_submit(
+            self.ptr, kernel_set, grid_size, float(cell_size_rad),
+            float(w_scale), n, *(a.ctypes.data for a in arrs), grid_id,
+            out.ctypes.data if out is not None else None, ct.byref(st))
+        _check(st, "gw_server_submit")
+        self._keep[rid] = (arrs, out)
+        return rid
+
+    def wait(self, rid, timeout=-1.0):
+        """Returns (RequestResult, grid or None); None on timeout."""
+        st = ct.c_int(0)
+        res = RequestResult()
+        done = lib().gw_server_wait(self.ptr, rid, float(timeout),
+                                    ct.byref(res), ct.byref(st))
+        _check(st, "gw_server_wait")
+        if not done:
+            return None
+        _, out = self._keep.pop(rid)
+        if res.status != 0:
+            raise GridderError(res.status, f"request {rid}")
+        return res, out
+
+    def stats(self):
+        s = ServerStats()
+        lib().gw_server_get_stats(self.ptr, ct.byref(s))
+        return {name: int(getattr(s, name)) for name, _ in s._fields_}
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().gw_server_free(self.ptr)
+            self.ptr = None
+            self._keep = {}
 
     def __del__(self):
         try:

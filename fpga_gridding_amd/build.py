@@ -20,13 +20,14 @@ LIB = os.path.join(HERE, "libfpga_gridding_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GW_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["gridder.hip", "capi.cpp"]
+SOURCES = ["gridder.hip", "capi.cpp", "server.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
     "-std=c++17",
     "-fPIC",
+    "-pthread",
     "-fvisibility=hidden",
     # Packed FP32 instructions stay off (see the device code's notes).
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -55,8 +56,8 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
-           *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread",
+           "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -11,6 +11,7 @@
 // kernel set, so repeated calls per visibility block (main.cpp:413-535)
 // upload the cube once.
 #include "gridder_internal.h"
+#include "server.h"
 
 #include "fpga_gridding_amd.h"
 
@@ -293,6 +294,11 @@ struct gw_plan
 struct gw_batch
 {
     gw::Batch* impl;
+};
+
+struct gw_server
+{
+    gw::Server* impl;
 };
 
 extern "C" {
@@ -739,6 +745,150 @@ GW_EXPORT double gw_batch_last_ms(gw_batch* batch)
     double ms = 0.0;
     guarded(&status, [&] { ms = batch->impl->last_ms(); });
     return ms;
+}
+
+GW_EXPORT gw_server* gw_server_create(size_t hbm_budget_bytes,
+        int max_batch_jobs, int num_streams, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return nullptr;
+    gw_server* out = nullptr;
+    guarded(status, [&] {
+        std::unique_ptr<gw::Server> sv(new gw::Server(hbm_budget_bytes,
+                max_batch_jobs, num_streams));
+        out = new gw_server{sv.release()};
+    });
+    return out;
+}
+
+GW_EXPORT void gw_server_free(gw_server* server)
+{
+    if (!server) return;
+    delete server->impl;
+    delete server;
+}
+
+GW_EXPORT int gw_server_add_kernel_set(gw_server* server, size_t num_w_planes,
+        const int* support, int oversample, int conv_size_half,
+        const float* conv_func, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return -1;
+    int id = -1;
+    guarded(status, [&] {
+        if (!server) gw::fail(gw::kErrInvalidArgument, "server is NULL");
+        id = server->impl->add_kernel_set(num_w_planes, support, oversample,
+                                          conv_size_half, conv_func);
+    });
+    return id;
+}
+
+GW_EXPORT int gw_server_create_grid(gw_server* server, int grid_size,
+        int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return -1;
+    int id = -1;
+    guarded(status, [&] {
+        if (!server) gw::fail(gw::kErrInvalidArgument, "server is NULL");
+        id = server->impl->create_grid(grid_size);
+    });
+    return id;
+}
+
+GW_EXPORT void gw_server_read_grid(gw_server* server, int grid_id,
+        float* grid, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!server || !grid)
+            gw::fail(gw::kErrInvalidArgument, "server and grid must not be "
+                     "NULL");
+        server->impl->read_grid(grid_id, grid);
+    });
+}
+
+GW_EXPORT void gw_server_free_grid(gw_server* server, int grid_id)
+{
+    if (!server) return;
+    int status = 0;
+    guarded(&status, [&] { server->impl->free_grid(grid_id); });
+}
+
+GW_EXPORT long long gw_server_submit(gw_server* server, int kernel_set,
+        int grid_size, double cell_size_rad, double w_scale,
+        size_t num_points, const float* uu, const float* vv, const float* ww,
+        const float* vis, int grid_id, float* grid_out, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return -1;
+    long long id = -1;
+    guarded(status, [&] {
+        if (!server) gw::fail(gw::kErrInvalidArgument, "server is NULL");
+        gw::Server::Request r;
+        r.kernel_set = kernel_set;
+        r.grid_size = grid_size;
+        r.cell_size_rad = cell_size_rad;
+        r.w_scale = w_scale;
+        r.n = num_points;
+        r.uu = uu;
+        r.vv = vv;
+        r.ww = ww;
+        r.vis = vis;
+        r.grid_id = grid_id;
+        r.grid_out = grid_out;
+        id = server->impl->submit(r);
+    });
+    return id;
+}
+
+GW_EXPORT int gw_server_wait(gw_server* server, long long request,
+        double timeout_s, gw_request_result* result, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return 0;
+    int done = 0;
+    guarded(status, [&] {
+        if (!server) gw::fail(gw::kErrInvalidArgument, "server is NULL");
+        gw::Server::Result r;
+        done = server->impl->wait(request, timeout_s, &r) ? 1 : 0;
+        if (done && result)
+        {
+            result->status = r.status;
+            result->num_skipped = r.skipped;
+            result->norm_delta = r.norm_delta;
+            result->num_updates = r.updates;
+            result->t_submit = r.t_submit;
+            result->t_admit = r.t_admit;
+            result->t_start = r.t_start;
+            result->t_done = r.t_done;
+        }
+    });
+    return done;
+}
+
+GW_EXPORT void gw_server_get_stats(gw_server* server, gw_server_stats* out)
+{
+    if (!server || !out) return;
+    int status = 0;
+    guarded(&status, [&] {
+        const gw::Server::Stats st = server->impl->stats();
+        out->completed = st.completed;
+        out->batches = st.batches;
+        out->max_batch = st.max_batch;
+        out->plans_evicted = st.plans_evicted;
+        out->points = st.points;
+        out->device_bytes = st.device_bytes;
+        out->resident_plans = st.resident_plans;
+        out->pending = st.pending;
+    });
 }
 
 GW_EXPORT void gw_compensated_resolve(const float* d_comp, size_t num_cells,

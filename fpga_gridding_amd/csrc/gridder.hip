@@ -1547,6 +1547,8 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
         etab_taps_ = etab_d.size() / 2;
     }
     d_etab_ = dmalloc<char>(etab_bytes);
+    table_bytes_ = etab_bytes + P_ * (sizeof(int) + sizeof(PlaneInfo)) +
+            normtab.size() * sizeof(double);
     d_support_ = dmalloc<int>(P_);
     d_pinfo_ = dmalloc<PlaneInfo>(P_);
     d_normtab_ = dmalloc<double>(normtab.size());
@@ -2512,6 +2514,32 @@ void Batch::results(int job, size_t* skipped, double* norm_delta,
     if (skipped) *skipped = (size_t)sm.skipped;
     if (norm_delta) *norm_delta = sm.norm_delta;
     if (updates) *updates = sm.updates;
+}
+
+size_t Batch::device_bytes() const
+{
+    const size_t items = cap_points_ + cap_points_ / kJobPoints + 2;
+    return cap_points_ * (2 * sizeof(unsigned long long) + 2 * sizeof(unsigned)
+                          + sizeof(RecF) + sizeof(int4) + sizeof(float4)) +
+           cap_tiles_ * 5 * sizeof(unsigned) + items * sizeof(int4) +
+           cap_jobs_ * (sizeof(BatchJob) + sizeof(Summary)) +
+           cap_combo_ * sizeof(unsigned);
+}
+
+size_t Batch::workspace_estimate(size_t points)
+{
+    const size_t pcap = std::max<size_t>(points, 1024);
+    return pcap * (2 * sizeof(unsigned long long) + 2 * sizeof(unsigned) +
+                   sizeof(RecF) + sizeof(int4) + sizeof(float4)) +
+           (pcap + pcap / kJobPoints + 2) * sizeof(int4) + (1u << 20);
+}
+
+void Batch::release_workspace()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graph();
+    free_buffers();
+    dirty_ = true;
 }
 
 double Batch::last_ms()

@@ -245,6 +245,8 @@ public:
     float grid_scale_f() const { return gscale_f_; }
     float w_scale_f() const { return ws_f_; }
     int grid_blocks() const { return grid_blocks_; }
+    // Device bytes of the plan's constant tables (kernel set).
+    size_t table_bytes() const { return table_bytes_; }
 
 private:
     void ensure_capacity(size_t n);
@@ -284,6 +286,7 @@ private:
 
     // Constant device tables.
     void* d_etab_ = nullptr;             // expanded taps (complex, T)
+    size_t table_bytes_ = 0;
     size_t etab_taps_ = 0;
     int* d_support_ = nullptr;
     PlaneInfo* d_pinfo_ = nullptr;
@@ -358,6 +361,15 @@ public:
                  unsigned long long* updates);
     // Device milliseconds of the last run / launch (waits).
     double last_ms();
+    // Device bytes held by the batch's workspace.
+    size_t device_bytes() const;
+    // Workspace bytes a batch of `points` points needs (tile and job
+    // tables approximated by a fixed allowance).
+    static size_t workspace_estimate(size_t points);
+    // Frees the workspace (reallocated by the next run / capture).
+    void release_workspace();
+    // Event recorded after the last run / launch (for polling).
+    hipEvent_t done_event() const { return ev_[1]; }
 
 private:
     struct Entry

@@ -206,6 +206,70 @@ GW_EXPORT void gw_batch_results(gw_batch* batch, int job, size_t* num_skipped,
 GW_EXPORT double gw_batch_last_ms(gw_batch* batch);
 
 /* ---------------------------------------------------------------------- */
+/* Serving: a request loop with admission by device-memory budget         */
+/* ---------------------------------------------------------------------- */
+/* A server owns a worker thread on the current HIP device.  Requests (the
+ * arguments of oskar_grid_wproj_f, host arrays) are queued FIFO; the worker
+ * admits them while the budget allows -- kernel-set tables (plans, cached
+ * per kernel set, grid side, cell size and w_scale; idle ones evicted least
+ * recently used first), resident grids, staging and batch workspace count
+ * against it -- and grids each admitted group as one batch (gw_batch_*) on
+ * one of several streams, so that requests arriving while others run start
+ * as soon as a stream and memory are free.  No reference counterpart: the
+ * reference grids one job per call (oskar_grid_wproj.c:128-243). */
+
+typedef struct gw_server gw_server;
+
+typedef struct gw_request_result
+{
+    int status;                  /* 0, or the request's error code */
+    size_t num_skipped;
+    double norm_delta;           /* this request's norm increment */
+    unsigned long long num_updates;
+    /* Steady-clock seconds: submitted, admitted, enqueued, completed. */
+    double t_submit, t_admit, t_start, t_done;
+} gw_request_result;
+
+typedef struct gw_server_stats
+{
+    size_t completed, batches, max_batch, plans_evicted, points;
+    size_t device_bytes;         /* held now (tables, grids, staging) */
+    size_t resident_plans, pending;
+} gw_server_stats;
+
+/* hbm_budget_bytes: device memory the server may hold; max_batch_jobs:
+ * most requests gridded together; num_streams: batches in flight (1-8). */
+GW_EXPORT gw_server* gw_server_create(size_t hbm_budget_bytes,
+        int max_batch_jobs, int num_streams, int* status);
+/* Finishes every queued request, then frees all device memory. */
+GW_EXPORT void gw_server_free(gw_server* server);
+/* Registers a W-kernel set (arguments as oskar_grid_wproj_f's, copied);
+ * returns its id. */
+GW_EXPORT int gw_server_add_kernel_set(gw_server* server, size_t num_w_planes,
+        const int* support, int oversample, int conv_size_half,
+        const float* conv_func, int* status);
+/* A zeroed resident grid (grid_size^2 complex64) that requests accumulate
+ * into; returns its id.  Reading waits until no queued request uses it. */
+GW_EXPORT int gw_server_create_grid(gw_server* server, int grid_size,
+        int* status);
+GW_EXPORT void gw_server_read_grid(gw_server* server, int grid_id,
+        float* grid, int* status);
+GW_EXPORT void gw_server_free_grid(gw_server* server, int grid_id);
+/* Queues a request; returns its id (> 0).  The host arrays must stay
+ * valid until the request has been waited for.  grid_id >= 0 adds into
+ * that resident grid; grid_id < 0 grids into a fresh zeroed grid that is
+ * copied to grid_out (grid_size^2 complex64) on completion. */
+GW_EXPORT long long gw_server_submit(gw_server* server, int kernel_set,
+        int grid_size, double cell_size_rad, double w_scale,
+        size_t num_points, const float* uu, const float* vv, const float* ww,
+        const float* vis, int grid_id, float* grid_out, int* status);
+/* Waits for a request (timeout_s < 0: without limit).  Returns 1 and fills
+ * *result once it is done (the request is then forgotten), 0 on timeout. */
+GW_EXPORT int gw_server_wait(gw_server* server, long long request,
+        double timeout_s, gw_request_result* result, int* status);
+GW_EXPORT void gw_server_get_stats(gw_server* server, gw_server_stats* out);
+
+/* ---------------------------------------------------------------------- */
 /* Device memory helpers (current HIP device)                              */
 /* ---------------------------------------------------------------------- */
 /* The copy and memset helpers first wait for all work on the device (any
