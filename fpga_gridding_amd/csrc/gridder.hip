@@ -1996,76 +1996,176 @@ static_assert(sizeof(PlaneInfo) == 32 && offsetof(PlaneInfo, eoff) == 24,
         "PlaneInfo word layout");
 constexpr unsigned long long kBatchLocalMask = (1ull << kBatchJobShift) - 1;
 
-// Points per global tile (skipped points, key == 1 << key_bits, excluded).
-__global__ void __launch_bounds__(256) k_tile_count(size_t n,
+// ------------------------------------------------------------------------
+// Batch ordering.  Every point's key becomes (global tile, in-tile key):
+//   (tile_base + tile) << lmax | (local key & in-tile bits)
+// with lmax the widest in-tile key of the batch; skipped points get
+// tiles_total << lmax and sort last.  A stable LSD radix sort of these keys
+// (8-bit digits, as many passes as the key has bits) gives each tile's
+// points in the order the single-job path sorts them (plane, position in
+// the tile, sub-cell table), so a batch job's work items are the same as
+// the job's own.  The sort is built from plain kernels and the scan above
+// (no memset, no host round trip), so the sequence can be captured.
+// ------------------------------------------------------------------------
+
+constexpr int kRadixThreads = 256;
+constexpr int kRadixRounds = 16;
+constexpr int kRadixChunk = kRadixThreads * kRadixRounds;
+
+__global__ void __launch_bounds__(256) k_batch_key(size_t n,
         const unsigned long long* __restrict__ keys,
-        const BatchJob* __restrict__ jobs, unsigned* __restrict__ count)
+        const BatchJob* __restrict__ jobs, int lmax,
+        unsigned long long sentinel, unsigned long long* __restrict__ out,
+        Summary* __restrict__ bsum)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) bsum->nv32 = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
     {
         const unsigned long long key = keys[i];
         const BatchJob& d = jobs[key >> kBatchJobShift];
         const unsigned long long local = key & kBatchLocalMask;
-        if (local >> d.key_bits) continue;
-        atomicAdd(&count[d.tile_base + (unsigned)(local >> d.tile_shift)], 1u);
+        unsigned long long c = sentinel;
+        if (!(local >> d.key_bits))
+            c = ((unsigned long long)(d.tile_base +
+                    (unsigned)(local >> d.tile_shift)) << lmax) |
+                (local & ((1ull << d.tile_shift) - 1));
+        out[i] = c;
     }
 }
 
-// Counting-sort scatter: each point to its tile's range (the order inside a
-// tile is restored by k_item_sort).
-__global__ void __launch_bounds__(256) k_tile_scatter(size_t n,
+// Digit histogram of each chunk of kRadixChunk keys, digit-major:
+// hist[digit * nblocks + block].
+__global__ void __launch_bounds__(kRadixThreads) k_radix_hist(size_t n,
+        const unsigned long long* __restrict__ keys, int shift,
+        unsigned nblocks, unsigned* __restrict__ hist)
+{
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t b0 = (size_t)blockIdx.x * kRadixChunk + threadIdx.x;
+#pragma unroll 4
+    for (int r = 0; r < kRadixRounds; ++r)
+    {
+        const size_t i = b0 + (size_t)r * kRadixThreads;
+        if (i < n) atomicAdd(&h[(unsigned)(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of each chunk by digit: offs is the exclusive scan of
+// hist; inside a chunk, keys keep their order (rounds in order, waves in
+// order inside a round, lanes in order inside a wave via ballot matching).
+__global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
         const unsigned long long* __restrict__ keys,
-        const unsigned* __restrict__ idx, const BatchJob* __restrict__ jobs,
-        const unsigned* __restrict__ tstart, unsigned* __restrict__ cursor,
+        const unsigned* __restrict__ vals, int shift, unsigned nblocks,
+        const unsigned* __restrict__ offs,
         unsigned long long* __restrict__ keys_out,
-        unsigned* __restrict__ idx_out)
+        unsigned* __restrict__ vals_out)
+{
+    constexpr int kWaves = kRadixThreads / 64;
+    __shared__ unsigned base[256];
+    __shared__ unsigned cnt[kWaves][256];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const unsigned long long below = (1ull << lane) - 1;
+    base[tid] = offs[(size_t)tid * nblocks + blockIdx.x];
+    const size_t b0 = (size_t)blockIdx.x * kRadixChunk + tid;
+    for (int r = 0; r < kRadixRounds; ++r)
+    {
+        const size_t i = b0 + (size_t)r * kRadixThreads;
+        const bool ok = i < n;
+        const unsigned long long k = ok ? keys[i] : 0ull;
+        const unsigned v = ok ? vals[i] : 0u;
+        const unsigned d = (unsigned)(k >> shift) & 255u;
+        unsigned long long peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+        {
+            const unsigned long long m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+#pragma unroll
+        for (int q = 0; q < kWaves; ++q) cnt[q][tid] = 0;
+        __syncthreads();
+        if (ok && (peers & below) == 0) cnt[w][d] = __popcll(peers);
+        __syncthreads();
+        if (ok)
+        {
+            unsigned pos = base[d] + __popcll(peers & below);
+            for (int q = 0; q < w; ++q) pos += cnt[q][d];
+            keys_out[pos] = k;
+            vals_out[pos] = v;
+        }
+        __syncthreads();
+        unsigned add = 0;
+#pragma unroll
+        for (int q = 0; q < kWaves; ++q) add += cnt[q][tid];
+        base[tid] += add;
+    }
+}
+
+// Tile ranges of the sorted keys: tstart[t] = first point, tend[t] = one
+// past the last (both zero for empty tiles); bsum->nv32 = points gridded.
+__global__ void __launch_bounds__(256) k_tile_bounds(size_t n,
+        const unsigned long long* __restrict__ keys, int lmax,
+        unsigned sentinel_tile, unsigned* __restrict__ tstart,
+        unsigned* __restrict__ tend, Summary* __restrict__ bsum)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
     {
-        const unsigned long long key = keys[i];
-        const BatchJob& d = jobs[key >> kBatchJobShift];
-        const unsigned long long local = key & kBatchLocalMask;
-        if (local >> d.key_bits) continue;
-        const unsigned t = d.tile_base + (unsigned)(local >> d.tile_shift);
-        const unsigned pos = tstart[t] + atomicAdd(&cursor[t], 1u);
-        keys_out[pos] = key;
-        idx_out[pos] = idx[i];
+        const unsigned t = (unsigned)(keys[i] >> lmax);
+        if (t == sentinel_tile) continue;
+        if (i == 0 || (unsigned)(keys[i - 1] >> lmax) != t)
+            tstart[t] = (unsigned)i;
+        const unsigned nt = i + 1 < n ? (unsigned)(keys[i + 1] >> lmax)
+                                      : sentinel_tile;
+        if (nt != t) tend[t] = (unsigned)(i + 1);
+        if (nt == sentinel_tile) bsum->nv32 = (unsigned)(i + 1);
     }
 }
 
 // Work items per tile: ceil(points / kJobPoints).
 __global__ void __launch_bounds__(256) k_tile_items(
-        const unsigned* __restrict__ ntiles, const unsigned* __restrict__ count,
-        unsigned* __restrict__ icnt)
+        const unsigned* __restrict__ ntiles, const unsigned* __restrict__ tstart,
+        const unsigned* __restrict__ tend, unsigned* __restrict__ icnt)
 {
     const unsigned nt = *ntiles;
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < nt;
          t += gridDim.x * blockDim.x)
-        icnt[t] = (count[t] + kJobPoints - 1) / kJobPoints;
+        icnt[t] = (tend[t] - tstart[t] + kJobPoints - 1) / kJobPoints;
+}
+
+// Job owning global tile t: the last with tile_base <= t.
+__device__ __forceinline__ int batch_job_of_tile(const BatchJob* jobs,
+                                                 int njobs, unsigned t)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_base <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // Work items = (tile origin u | v << 16, first sorted point, points, job).
 __global__ void __launch_bounds__(256) k_write_tile_items(unsigned ntiles,
-        const unsigned* __restrict__ count, const unsigned* __restrict__ tstart,
+        const unsigned* __restrict__ tstart, const unsigned* __restrict__ tend,
         const unsigned* __restrict__ ioff, const BatchJob* __restrict__ jobs,
         int njobs, unsigned items_cap, int4* __restrict__ items)
 {
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles;
          t += gridDim.x * blockDim.x)
     {
-        const unsigned c = count[t];
+        const unsigned c = tend[t] - tstart[t];
         if (c == 0) continue;
-        // Job owning global tile t: the last with tile_base <= t.
-        int lo = 0, hi = njobs - 1;
-        while (lo < hi)
-        {
-            const int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].tile_base <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const BatchJob& d = jobs[lo];
+        const int j = batch_job_of_tile(jobs, njobs, t);
+        const BatchJob& d = jobs[j];
         const unsigned lt = t - d.tile_base;
         const unsigned ux0 = compact_bits16(lt) << d.tile_bits;
         const unsigned vy0 = compact_bits16((unsigned long long)lt >> 1)
@@ -2075,60 +2175,7 @@ __global__ void __launch_bounds__(256) k_write_tile_items(unsigned ntiles,
         for (unsigned first = 0; first < c; first += kJobPoints, ++o)
             if (o < items_cap)
                 items[o] = make_int4(origin, (int)(tstart[t] + first),
-                        (int)min((unsigned)kJobPoints, c - first), lo);
-    }
-}
-
-// Each work item's points sorted by key (job, tile, plane, position inside
-// the tile, sub-cell table) with a bitonic sort in LDS.
-__global__ void __launch_bounds__(1024) k_item_sort(
-        const Summary* __restrict__ bsum, const int4* __restrict__ items,
-        unsigned long long* __restrict__ keys, unsigned* __restrict__ idx)
-{
-    __shared__ unsigned long long sk[kJobPoints];
-    __shared__ unsigned si[kJobPoints];
-    const int tid = threadIdx.x;
-    const unsigned total = bsum->total_items;
-    for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
-    {
-        const int4 item = items[it];
-        const int start = item.y, len = item.z;
-        for (int i = tid; i < kJobPoints; i += blockDim.x)
-        {
-            sk[i] = i < len ? keys[start + i] : ~0ull;
-            si[i] = i < len ? idx[start + i] : 0u;
-        }
-        __syncthreads();
-        for (int k = 2; k <= kJobPoints; k <<= 1)
-        {
-            for (int j = k >> 1; j > 0; j >>= 1)
-            {
-                for (int i = tid; i < kJobPoints; i += blockDim.x)
-                {
-                    const int l = i ^ j;
-                    if (l > i)
-                    {
-                        const bool up = (i & k) == 0;
-                        const unsigned long long a = sk[i], b = sk[l];
-                        if ((a > b) == up)
-                        {
-                            sk[i] = b;
-                            sk[l] = a;
-                            const unsigned t = si[i];
-                            si[i] = si[l];
-                            si[l] = t;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        for (int i = tid; i < len; i += blockDim.x)
-        {
-            keys[start + i] = sk[i];
-            idx[start + i] = si[i];
-        }
-        __syncthreads();
+                        (int)min((unsigned)kJobPoints, c - first), j);
     }
 }
 
@@ -2137,14 +2184,15 @@ __global__ void __launch_bounds__(256) k_gather_batch(
         const Summary* __restrict__ bsum,
         const unsigned long long* __restrict__ keys,
         const unsigned* __restrict__ idx, const RecF* __restrict__ rec,
-        const BatchJob* __restrict__ jobs, int4* __restrict__ meta,
-        float4* __restrict__ vq)
+        const BatchJob* __restrict__ jobs, int njobs, int lmax,
+        int4* __restrict__ meta, float4* __restrict__ vq)
 {
     const unsigned nv = bsum->nv32;
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
          s += gridDim.x * blockDim.x)
     {
-        const BatchJob& d = jobs[keys[s] >> kBatchJobShift];
+        const BatchJob& d = jobs[batch_job_of_tile(jobs, njobs,
+                (unsigned)(keys[s] >> lmax))];
         const RecF r = rec[idx[s]];
         const int plane = (unsigned)r.sub >> 17;
         const float sgn = ((r.sub >> 16) & 1) ? -1.0f : 1.0f;
@@ -2216,7 +2264,7 @@ Batch::Batch()
     for (auto& e : ev_)
         check_hip(hipEventCreate(&e), kErrComputeDevices, "hipEventCreate");
     d_bsum_ = dmalloc<Summary>(1);
-    d_ntiles_ = dmalloc<unsigned>(1);
+    d_ntiles_ = dmalloc<unsigned>(2);
 }
 
 Batch::~Batch()
@@ -2244,9 +2292,9 @@ void Batch::free_buffers()
     for (void* p : {(void*)d_desc_, (void*)d_sum_, (void*)d_combo_,
                     (void*)d_keys_[0], (void*)d_keys_[1], (void*)d_idx_[0],
                     (void*)d_idx_[1], d_rec_, (void*)d_meta_, d_vq_,
-                    (void*)d_tcount_, (void*)d_tstart_, (void*)d_tcursor_,
-                    (void*)d_icnt_, (void*)d_ioff_, (void*)d_part_,
-                    (void*)d_items_})
+                    (void*)d_tstart_, (void*)d_tend_, (void*)d_icnt_,
+                    (void*)d_ioff_, (void*)d_part_, (void*)d_rhist_,
+                    (void*)d_roff_, (void*)d_items_})
         dfree(p);
     if (h_sum_) (void)hipHostFree(h_sum_);
     d_desc_ = nullptr;
@@ -2257,7 +2305,8 @@ void Batch::free_buffers()
     d_rec_ = nullptr;
     d_meta_ = nullptr;
     d_vq_ = nullptr;
-    d_tcount_ = d_tstart_ = d_tcursor_ = d_icnt_ = d_ioff_ = d_part_ = nullptr;
+    d_tstart_ = d_tend_ = d_icnt_ = d_ioff_ = d_part_ = nullptr;
+    d_rhist_ = d_roff_ = nullptr;
     d_items_ = nullptr;
     h_sum_ = nullptr;
     cap_points_ = cap_tiles_ = cap_combo_ = cap_jobs_ = 0;
@@ -2338,6 +2387,18 @@ void Batch::prepare()
     n_total_ = pts;
     tiles_total_ = tiles;
     combo_total_ = combos;
+    // Batch key: global tile above the widest in-tile key; the sentinel
+    // tile (skipped points) is tiles_total.
+    lmax_ = 0;
+    for (const BatchJob& d : desc_) lmax_ = std::max(lmax_, d.tile_shift);
+    int tbits = 0;
+    while ((tiles >> tbits) != 0) ++tbits;
+    radix_bits_ = lmax_ + tbits;
+    if (radix_bits_ > 64)
+        fail(kErrInvalidArgument, "batch key exceeds 64 bits");
+    // The previous run may still read the job table.
+    check_hip(hipStreamSynchronize(stream_), kErrKernelLaunch,
+            "stream synchronize");
     const size_t pcap = std::max<size_t>(pts, 1024);
     items_cap_ = pcap + pcap / kJobPoints + 2;
     // Allocated on first use even for an empty batch: the scans write
@@ -2359,9 +2420,12 @@ void Batch::prepare()
         d_rec_ = dmalloc<RecF>(cap_points_);
         d_meta_ = dmalloc<int4>(cap_points_);
         d_vq_ = dmalloc<float4>(cap_points_);
-        d_tcount_ = dmalloc<unsigned>(cap_tiles_);
         d_tstart_ = dmalloc<unsigned>(cap_tiles_);
-        d_tcursor_ = dmalloc<unsigned>(cap_tiles_);
+        d_tend_ = dmalloc<unsigned>(cap_tiles_);
+        const size_t rh = 256 * ((cap_points_ + kRadixChunk - 1) /
+                                 kRadixChunk);
+        d_rhist_ = dmalloc<unsigned>(rh);
+        d_roff_ = dmalloc<unsigned>(rh);
         d_icnt_ = dmalloc<unsigned>(cap_tiles_);
         d_ioff_ = dmalloc<unsigned>(cap_tiles_);
         d_part_ = dmalloc<unsigned>(
@@ -2377,8 +2441,9 @@ void Batch::prepare()
     if (J)
         check_hip(hipMemcpy(d_desc_, desc_.data(), J * sizeof(BatchJob),
                 hipMemcpyHostToDevice), kErrMemCopy, "upload job table");
-    const unsigned nt = (unsigned)tiles;
-    check_hip(hipMemcpy(d_ntiles_, &nt, sizeof(unsigned),
+    const unsigned counts[2] = {(unsigned)tiles, (unsigned)(256 *
+            ((pts + kRadixChunk - 1) / kRadixChunk))};
+    check_hip(hipMemcpy(d_ntiles_, counts, sizeof(counts),
             hipMemcpyHostToDevice), kErrMemCopy, "upload tile count");
 }
 
@@ -2419,36 +2484,55 @@ void Batch::enqueue()
                 pl.device_normtab(), d_sum_ + j);
     }
     check_hip(hipGetLastError(), kErrKernelLaunch, "batch index launch");
-    fill_zero_u32(d_tcount_, tiles_total_, s);
-    fill_zero_u32(d_tcursor_, tiles_total_, s);
     const int pblocks = (int)std::max<size_t>(1, std::min<size_t>(
             (n_total_ + 255) / 256, (size_t)n_cu_ * 8));
     const int tblocks = (int)std::max<size_t>(1, std::min<size_t>(
             (tiles_total_ + 255) / 256, (size_t)n_cu_ * 8));
-    hipLaunchKernelGGL(k_tile_count, dim3(pblocks), dim3(256), 0, s,
+    const unsigned long long sentinel =
+            (unsigned long long)tiles_total_ << lmax_;
+    fill_zero_u32(d_tstart_, tiles_total_, s);
+    fill_zero_u32(d_tend_, tiles_total_, s);
+    hipLaunchKernelGGL(k_batch_key, dim3(pblocks), dim3(256), 0, s,
             n_total_, (const unsigned long long*)d_keys_[0],
-            (const BatchJob*)d_desc_, d_tcount_);
-    scan_u32_enqueue(d_tcount_, d_tstart_, d_ntiles_, tiles_total_,
-            &d_bsum_->nv32, d_part_, s);
-    hipLaunchKernelGGL(k_tile_scatter, dim3(pblocks), dim3(256), 0, s,
-            n_total_, (const unsigned long long*)d_keys_[0],
-            (const unsigned*)d_idx_[0], (const BatchJob*)d_desc_,
-            (const unsigned*)d_tstart_, d_tcursor_, d_keys_[1], d_idx_[1]);
+            (const BatchJob*)d_desc_, lmax_, sentinel, d_keys_[1], d_bsum_);
+    // Stable LSD radix sort of (batch key, point index).
+    unsigned long long* kin = d_keys_[1];
+    unsigned long long* kout = d_keys_[0];
+    unsigned* vin = d_idx_[0];
+    unsigned* vout = d_idx_[1];
+    const unsigned nchunk = (unsigned)((n_total_ + kRadixChunk - 1) /
+                                       kRadixChunk);
+    for (int shift = 0; nchunk && shift < radix_bits_; shift += 8)
+    {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nchunk), dim3(kRadixThreads),
+                0, s, n_total_, (const unsigned long long*)kin, shift, nchunk,
+                d_rhist_);
+        scan_u32_enqueue(d_rhist_, d_roff_, d_ntiles_ + 1,
+                256 * (size_t)nchunk, &d_bsum_->n_buckets, d_part_, s);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nchunk),
+                dim3(kRadixThreads), 0, s, n_total_,
+                (const unsigned long long*)kin, (const unsigned*)vin, shift,
+                nchunk, (const unsigned*)d_roff_, kout, vout);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    hipLaunchKernelGGL(k_tile_bounds, dim3(pblocks), dim3(256), 0, s,
+            n_total_, (const unsigned long long*)kin, lmax_,
+            (unsigned)tiles_total_, d_tstart_, d_tend_, d_bsum_);
     hipLaunchKernelGGL(k_tile_items, dim3(tblocks), dim3(256), 0, s,
-            (const unsigned*)d_ntiles_, (const unsigned*)d_tcount_, d_icnt_);
+            (const unsigned*)d_ntiles_, (const unsigned*)d_tstart_,
+            (const unsigned*)d_tend_, d_icnt_);
     scan_u32_enqueue(d_icnt_, d_ioff_, d_ntiles_, tiles_total_,
             &d_bsum_->total_items, d_part_, s);
     hipLaunchKernelGGL(k_write_tile_items, dim3(tblocks), dim3(256), 0, s,
-            (unsigned)tiles_total_, (const unsigned*)d_tcount_,
-            (const unsigned*)d_tstart_, (const unsigned*)d_ioff_,
+            (unsigned)tiles_total_, (const unsigned*)d_tstart_,
+            (const unsigned*)d_tend_, (const unsigned*)d_ioff_,
             (const BatchJob*)d_desc_, (int)J, (unsigned)items_cap_, d_items_);
-    hipLaunchKernelGGL(k_item_sort, dim3(grid_blocks_), dim3(1024), 0, s,
-            (const Summary*)d_bsum_, (const int4*)d_items_, d_keys_[1],
-            d_idx_[1]);
     hipLaunchKernelGGL(k_gather_batch, dim3(pblocks), dim3(256), 0, s,
-            (const Summary*)d_bsum_, (const unsigned long long*)d_keys_[1],
-            (const unsigned*)d_idx_[1], (const RecF*)d_rec_,
-            (const BatchJob*)d_desc_, d_meta_, (float4*)d_vq_);
+            (const Summary*)d_bsum_, (const unsigned long long*)kin,
+            (const unsigned*)vin, (const RecF*)d_rec_,
+            (const BatchJob*)d_desc_, (int)J, lmax_, d_meta_,
+            (float4*)d_vq_);
     hipLaunchKernelGGL(k_grid_batch, dim3(grid_blocks_), dim3(kGridThreads),
             0, s, (const Summary*)d_bsum_, (const int4*)d_items_,
             (const BatchJob*)d_desc_, (const int4*)d_meta_,

@@ -401,13 +401,16 @@ private:
     void* d_rec_ = nullptr;
     int4* d_meta_ = nullptr;
     void* d_vq_ = nullptr;
-    unsigned* d_tcount_ = nullptr;
-    unsigned* d_tstart_ = nullptr;
-    unsigned* d_tcursor_ = nullptr;
+    unsigned* d_tstart_ = nullptr;    // per global tile: first point
+    unsigned* d_tend_ = nullptr;      // one past the last
     unsigned* d_icnt_ = nullptr;
     unsigned* d_ioff_ = nullptr;
     unsigned* d_part_ = nullptr;
-    unsigned* d_ntiles_ = nullptr;
+    unsigned* d_rhist_ = nullptr;     // radix digit counts per chunk
+    unsigned* d_roff_ = nullptr;      // their exclusive scan
+    unsigned* d_ntiles_ = nullptr;    // [tiles, radix histogram length]
+    int lmax_ = 0;                    // in-tile key bits of the batch key
+    int radix_bits_ = 0;              // significant bits of the batch key
     int4* d_items_ = nullptr;
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;

@@ -258,7 +258,10 @@ def test_round_half_away_from_zero():
     rec, _ = oracle.locate_f(wl)
     gs = np.float32(wl.grid_size) * np.float32(wl.cell_size_rad)
     pu = -wl.uu.astype(np.float64) * float(gs)
-    tie = np.isfinite(pu) & (np.abs(pu - np.floor(pu) - 0.5) == 0)
+    finite = np.isfinite(pu)
+    frac = np.full_like(pu, np.nan)
+    frac[finite] = pu[finite] - np.floor(pu[finite])
+    tie = finite & (np.abs(frac - 0.5) == 0)
     assert tie.sum() >= 10
     expect = np.sign(pu[tie]) * np.floor(np.abs(pu[tie]) + 0.5)
     assert np.array_equal(rec["grid_u"][tie] - wl.grid_size // 2,
