@@ -44,6 +44,9 @@ EXPORTED_SYMBOLS = (
     "gw_server_create", "gw_server_free", "gw_server_add_kernel_set",
     "gw_server_create_grid", "gw_server_read_grid", "gw_server_free_grid",
     "gw_server_submit", "gw_server_wait", "gw_server_get_stats",
+    "oskar_grid_weights_write_f", "oskar_grid_weights_read_f",
+    "oskar_grid_weights_write_d", "oskar_grid_weights_read_d",
+    "gw_grid_weights_write_device", "gw_grid_weights_read_device",
 )
 
 
@@ -197,6 +200,21 @@ def lib():
     L.gw_server_wait.restype = ct.c_int
     L.gw_server_get_stats.argtypes = [_vp, ct.POINTER(ServerStats)]
     L.gw_server_get_stats.restype = None
+    for sfx, fp, real in (("f", _f32p, ct.c_float), ("d", _f64p, ct.c_double)):
+        w = getattr(L, "oskar_grid_weights_write_" + sfx)
+        w.argtypes = [_sz, fp, fp, fp, real, ct.c_int, _szp, fp]
+        w.restype = None
+        r = getattr(L, "oskar_grid_weights_read_" + sfx)
+        r.argtypes = [_sz, fp, fp, fp, fp, real, ct.c_int, _szp, fp]
+        r.restype = None
+    L.gw_grid_weights_write_device.argtypes = [
+        ct.c_int, _sz, _vp, _vp, _vp, ct.c_double, ct.c_int, _szp, _vp, _vp,
+        _ip]
+    L.gw_grid_weights_write_device.restype = None
+    L.gw_grid_weights_read_device.argtypes = [
+        ct.c_int, _sz, _vp, _vp, _vp, _vp, ct.c_double, ct.c_int, _szp, _vp,
+        _vp, _ip]
+    L.gw_grid_weights_read_device.restype = None
     _lib = L
     return L
 
@@ -352,6 +370,58 @@ def grid_wproj_f_diagnostics(num_w_planes, support, oversample,
     if st:
         raise GridderError(st, "oskar_grid_wproj_f_diagnostics")
     return int(sk.value), float(nrm.value)
+
+
+def grid_weights_write(uu, vv, weight, cell_size_rad, grid_size, grid,
+                       double=False):
+    """oskar_grid_weights_write_f / _d on host arrays: adds into ``grid``
+    (float32 or float64 [grid_size^2], modified in place); returns
+    num_skipped."""
+    dt = np.float64 if double else np.float32
+    if grid.dtype != dt or not grid.flags["C_CONTIGUOUS"]:
+        raise ValueError("grid must be a contiguous array of the call's type")
+    if grid.size != grid_size * grid_size:
+        raise ValueError("grid must hold grid_size^2 values")
+    L = lib()
+    fn = L.oskar_grid_weights_write_d if double else \
+        L.oskar_grid_weights_write_f
+    sk = ct.c_size_t(0)
+    uu, vv, w = _c(uu, dt), _c(vv, dt), _c(weight, dt)
+    if not (uu.size == vv.size == w.size):
+        raise ValueError("uu, vv and weight need the same length")
+    L.oskar_grid_wproj_last_error()
+    fn(uu.size, uu, vv, w, cell_size_rad, grid_size, ct.byref(sk), grid)
+    st = L.oskar_grid_wproj_last_error()
+    if st:
+        raise GridderError(st, fn.__name__)
+    return int(sk.value)
+
+
+def grid_weights_read(uu, vv, weight_in, weight_out, cell_size_rad,
+                      grid_size, grid, double=False):
+    """oskar_grid_weights_read_f / _d on host arrays: fills ``weight_out``
+    (modified in place, skipped points untouched); returns num_skipped."""
+    dt = np.float64 if double else np.float32
+    if weight_out.dtype != dt or not weight_out.flags["C_CONTIGUOUS"]:
+        raise ValueError("weight_out must be a contiguous array of the "
+                         "call's type")
+    L = lib()
+    fn = L.oskar_grid_weights_read_d if double else \
+        L.oskar_grid_weights_read_f
+    sk = ct.c_size_t(0)
+    uu, vv, w = _c(uu, dt), _c(vv, dt), _c(weight_in, dt)
+    if not (uu.size == vv.size == w.size == weight_out.size):
+        raise ValueError("uu, vv, weight_in and weight_out need the same "
+                         "length")
+    g = _c(grid, dt)
+    if g.size != grid_size * grid_size:
+        raise ValueError("grid must hold grid_size^2 values")
+    fn(uu.size, uu, vv, w, weight_out, cell_size_rad, grid_size,
+       ct.byref(sk), g)
+    st = L.oskar_grid_wproj_last_error()
+    if st:
+        raise GridderError(st, fn.__name__)
+    return int(sk.value)
 
 
 def grid_workload_f(wl, grid=None, norm=0.0):

@@ -14,6 +14,7 @@
 #include "server.h"
 
 #include "fpga_gridding_amd.h"
+#include "oskar_grid_weights.h"
 
 #include <algorithm>
 #include <cstring>
@@ -73,6 +74,21 @@ struct DropinCache
     std::vector<char> cube;
     int device = -1;
     DevBuf in, grid, d_support_hist, d_plane_hist, d_cell_count, d_cell_wsum;
+    // Uniform weighting.
+    std::unique_ptr<gw::Weights> weights;
+    int weights_device = -1;
+    DevBuf w_in, w_grid;
+
+    gw::Weights& weights_for(int dev)
+    {
+        if (!weights || weights_device != dev)
+        {
+            weights.reset();
+            weights.reset(new gw::Weights());
+            weights_device = dev;
+        }
+        return *weights;
+    }
 
     bool matches(int prec_, size_t P_, const int* support_, int ov_, int C_,
                  const void* conv, size_t cube_bytes, double cell_,
@@ -92,6 +108,103 @@ DropinCache& cache()
 {
     if (!t_cache) t_cache = new DropinCache();
     return *t_cache;
+}
+
+int current_device()
+{
+    int dev = 0;
+    gw::check_hip(hipGetDevice(&dev), gw::kErrComputeDevices, "hipGetDevice");
+    return dev;
+}
+
+// oskar_grid_weights_write_* on host arrays: points and the caller's grid
+// uploaded, the grid accumulated on the device and copied back.
+template <typename T>
+void weights_write_host(int prec, size_t n, const T* uu, const T* vv,
+                        const T* weight, T cell, int G, size_t* num_skipped,
+                        T* grid)
+{
+    if (!num_skipped || !grid)
+        gw::fail(gw::kErrInvalidArgument, "num_skipped and grid must not be "
+                 "NULL");
+    if (n && (!uu || !vv || !weight))
+        gw::fail(gw::kErrInvalidArgument, "uu, vv and weight must not be "
+                 "NULL");
+    if (G < 1 || G > 65535)
+        gw::fail(gw::kErrInvalidArgument, "grid_size must be in [1, 65535]");
+    DropinCache& c = cache();
+    gw::Weights& w = c.weights_for(current_device());
+    const hipStream_t s = w.stream();
+    const size_t gbytes = (size_t)G * G * sizeof(T);
+    T* d_in = static_cast<T*>(c.w_in.get(3 * std::max<size_t>(n, 1) *
+                                         sizeof(T)));
+    T* d_grid = static_cast<T*>(c.w_grid.get(gbytes));
+    if (n)
+    {
+        gw::check_hip(hipMemcpyAsync(d_in, uu, n * sizeof(T),
+                hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload uu");
+        gw::check_hip(hipMemcpyAsync(d_in + n, vv, n * sizeof(T),
+                hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload vv");
+        gw::check_hip(hipMemcpyAsync(d_in + 2 * n, weight, n * sizeof(T),
+                hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload weight");
+    }
+    gw::check_hip(hipMemcpyAsync(d_grid, grid, gbytes, hipMemcpyHostToDevice,
+            s), gw::kErrMemCopy, "upload grid");
+    const size_t sk = w.write(prec, n, d_in, d_in + n, d_in + 2 * n,
+                              (double)cell, G, d_grid);
+    gw::check_hip(hipMemcpyAsync(grid, d_grid, gbytes, hipMemcpyDeviceToHost,
+            s), gw::kErrMemCopy, "download grid");
+    gw::check_hip(hipStreamSynchronize(s), gw::kErrKernelLaunch,
+            "stream synchronize");
+    *num_skipped = sk;
+}
+
+// oskar_grid_weights_read_*: weight_out is uploaded too, so that entries of
+// skipped points come back unchanged.
+template <typename T>
+void weights_read_host(int prec, size_t n, const T* uu, const T* vv,
+                       const T* w_in, T* w_out, T cell, int G,
+                       size_t* num_skipped, const T* grid)
+{
+    if (!num_skipped || !grid)
+        gw::fail(gw::kErrInvalidArgument, "num_skipped and grid must not be "
+                 "NULL");
+    if (n && (!uu || !vv || !w_in || !w_out))
+        gw::fail(gw::kErrInvalidArgument, "uu, vv, weight_in and weight_out "
+                 "must not be NULL");
+    if (G < 1 || G > 65535)
+        gw::fail(gw::kErrInvalidArgument, "grid_size must be in [1, 65535]");
+    DropinCache& c = cache();
+    gw::Weights& w = c.weights_for(current_device());
+    const hipStream_t s = w.stream();
+    const size_t gbytes = (size_t)G * G * sizeof(T);
+    T* d_in = static_cast<T*>(c.w_in.get(4 * std::max<size_t>(n, 1) *
+                                         sizeof(T)));
+    T* d_grid = static_cast<T*>(c.w_grid.get(gbytes));
+    if (n)
+    {
+        gw::check_hip(hipMemcpyAsync(d_in, uu, n * sizeof(T),
+                hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload uu");
+        gw::check_hip(hipMemcpyAsync(d_in + n, vv, n * sizeof(T),
+                hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload vv");
+        gw::check_hip(hipMemcpyAsync(d_in + 2 * n, w_in, n * sizeof(T),
+                hipMemcpyHostToDevice, s), gw::kErrMemCopy,
+                "upload weight_in");
+        gw::check_hip(hipMemcpyAsync(d_in + 3 * n, w_out, n * sizeof(T),
+                hipMemcpyHostToDevice, s), gw::kErrMemCopy,
+                "upload weight_out");
+    }
+    gw::check_hip(hipMemcpyAsync(d_grid, grid, gbytes, hipMemcpyHostToDevice,
+            s), gw::kErrMemCopy, "upload grid");
+    const size_t sk = w.read(prec, n, d_in, d_in + n, d_in + 2 * n,
+                             d_in + 3 * n, (double)cell, G, d_grid);
+    if (n)
+        gw::check_hip(hipMemcpyAsync(w_out, d_in + 3 * n, n * sizeof(T),
+                hipMemcpyDeviceToHost, s), gw::kErrMemCopy,
+                "download weight_out");
+    gw::check_hip(hipStreamSynchronize(s), gw::kErrKernelLaunch,
+            "stream synchronize");
+    *num_skipped = sk;
 }
 
 // Caller-side diagnostics arrays of oskar_grid_wproj_f_diagnostics.
@@ -440,6 +553,10 @@ GW_EXPORT void gw_release_cached(void)
     if (t_cache)
     {
         t_cache->plan.reset();
+        t_cache->weights.reset();
+        t_cache->weights_device = -1;
+        t_cache->w_in.release();
+        t_cache->w_grid.release();
         t_cache->in.release();
         t_cache->grid.release();
         t_cache->d_support_hist.release();
@@ -745,6 +862,105 @@ GW_EXPORT double gw_batch_last_ms(gw_batch* batch)
     double ms = 0.0;
     guarded(&status, [&] { ms = batch->impl->last_ms(); });
     return ms;
+}
+
+GW_EXPORT
+void oskar_grid_weights_write_f(const size_t num_points, const float* uu,
+        const float* vv, const float* weight, const float cell_size_rad,
+        const int grid_size, size_t* num_skipped, float* grid)
+{
+    int status = 0;
+    t_last_error = 0;
+    t_last_detail.clear();
+    guarded(&status, [&] {
+        weights_write_host<float>(GW_SINGLE, num_points, uu, vv, weight,
+                cell_size_rad, grid_size, num_skipped, grid);
+    });
+    t_last_error = status;
+}
+
+GW_EXPORT
+void oskar_grid_weights_write_d(const size_t num_points, const double* uu,
+        const double* vv, const double* weight, const double cell_size_rad,
+        const int grid_size, size_t* num_skipped, double* grid)
+{
+    int status = 0;
+    t_last_error = 0;
+    t_last_detail.clear();
+    guarded(&status, [&] {
+        weights_write_host<double>(GW_DOUBLE, num_points, uu, vv, weight,
+                cell_size_rad, grid_size, num_skipped, grid);
+    });
+    t_last_error = status;
+}
+
+GW_EXPORT
+void oskar_grid_weights_read_f(const size_t num_points, const float* uu,
+        const float* vv, const float* weight_in, float* weight_out,
+        const float cell_size_rad, const int grid_size,
+        size_t* num_skipped, const float* grid)
+{
+    int status = 0;
+    t_last_error = 0;
+    t_last_detail.clear();
+    guarded(&status, [&] {
+        weights_read_host<float>(GW_SINGLE, num_points, uu, vv, weight_in,
+                weight_out, cell_size_rad, grid_size, num_skipped, grid);
+    });
+    t_last_error = status;
+}
+
+GW_EXPORT
+void oskar_grid_weights_read_d(const size_t num_points, const double* uu,
+        const double* vv, const double* weight_in, double* weight_out,
+        const double cell_size_rad, const int grid_size,
+        size_t* num_skipped, const double* grid)
+{
+    int status = 0;
+    t_last_error = 0;
+    t_last_detail.clear();
+    guarded(&status, [&] {
+        weights_read_host<double>(GW_DOUBLE, num_points, uu, vv, weight_in,
+                weight_out, cell_size_rad, grid_size, num_skipped, grid);
+    });
+    t_last_error = status;
+}
+
+GW_EXPORT void gw_grid_weights_write_device(int precision, size_t num_points,
+        const void* d_uu, const void* d_vv, const void* d_weight,
+        double cell_size_rad, int grid_size, size_t* num_skipped,
+        void* d_grid, void* hip_stream, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!num_skipped)
+            gw::fail(gw::kErrInvalidArgument, "num_skipped must not be NULL");
+        gw::Weights& w = cache().weights_for(current_device());
+        *num_skipped = w.write(precision, num_points, d_uu, d_vv, d_weight,
+                cell_size_rad, grid_size, d_grid,
+                static_cast<hipStream_t>(hip_stream));
+    });
+}
+
+GW_EXPORT void gw_grid_weights_read_device(int precision, size_t num_points,
+        const void* d_uu, const void* d_vv, const void* d_weight_in,
+        void* d_weight_out, double cell_size_rad, int grid_size,
+        size_t* num_skipped, const void* d_grid, void* hip_stream,
+        int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!num_skipped)
+            gw::fail(gw::kErrInvalidArgument, "num_skipped must not be NULL");
+        gw::Weights& w = cache().weights_for(current_device());
+        *num_skipped = w.read(precision, num_points, d_uu, d_vv, d_weight_in,
+                d_weight_out, cell_size_rad, grid_size, d_grid,
+                static_cast<hipStream_t>(hip_stream));
+    });
 }
 
 GW_EXPORT gw_server* gw_server_create(size_t hbm_budget_bytes,

@@ -20,6 +20,7 @@
 //              the tile's grid region, added to the grid in HBM once per job.
 
 #include "gridder_internal.h"
+#include "host_semantics.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -42,44 +43,6 @@ void check_hip(hipError_t e, int code, const char* what)
     if (e != hipSuccess)
         fail(code, std::string(what) + ": " + hipGetErrorName(e) + " (" +
                 hipGetErrorString(e) + ")");
-}
-
-// -------------------------------------------------------------------------
-// Scalar helpers with the reference's (x86-64 host) semantics
-// -------------------------------------------------------------------------
-
-__device__ __forceinline__ float d_round(float x) { return roundf(x); }
-__device__ __forceinline__ double d_round(double x) { return round(x); }
-__device__ __forceinline__ float d_sqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double d_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float d_fabs(float x) { return fabsf(x); }
-__device__ __forceinline__ double d_fabs(double x) { return fabs(x); }
-
-// (int)x of an integral-valued x: cvttss2si / cvttsd2si return INT_MIN for
-// NaN and for values outside the int range.
-__device__ __forceinline__ int to_int_x86(float x)
-{
-    return (x >= -2147483648.0f && x < 2147483648.0f) ? (int)x : INT_MIN;
-}
-__device__ __forceinline__ int to_int_x86(double x)
-{
-    return (x >= -2147483648.0 && x < 2147483648.0) ? (int)x : INT_MIN;
-}
-
-// (size_t)x for x >= 0 or NaN, as gcc lowers it on x86-64: values below
-// 2^63 convert directly, NaN gives 2^63, larger values go through
-// x - 2^63 with bit 63 flipped, and the indefinite result of an out-of-range
-// x - 2^63 (e.g. +inf) flips to 0.
-template <typename T>
-__device__ __forceinline__ unsigned long long to_size_x86(T x)
-{
-    const T two63 = (T)9223372036854775808.0;
-    if (x < two63) return (unsigned long long)(long long)x;
-    if (x != x) return 0x8000000000000000ull;
-    const T t = x - two63;
-    if (t < two63)
-        return ((unsigned long long)(long long)t) ^ 0x8000000000000000ull;
-    return 0ull;
 }
 
 // Bits needed to hold values 0..n-1 (at least 1).
@@ -794,12 +757,12 @@ __device__ __forceinline__ void settle(T& v)
 // [cb, cb + 64 * wpp * NS), NS cells per lane.  LDS selects the LDS region
 // (else the grid in HBM).
 //
-// Arithmetic is FP64 from the products on (gfx950 issues FP64 FMA at the
-// rate of unpacked FP32 FMA): the current taps are held as double, a
-// product of two floats is exact in double, and the sums need no
-// conversion on their way into the FP64 region.  The first point of a
-// position run writes its products with a multiply, so a flush leaves the
-// sums as they are instead of clearing them.
+// Arithmetic: products and the sums of a position run in T (FP32 for the
+// float and 16-bit plans, FP64 for the double plan; kSumF64 switches the
+// float sums to FP64, at fewer cells per lane), widened to FP64 when a run
+// ends and is added into the FP64 region.  The first point of a position
+// run writes its products with a multiply, so a flush leaves the sums as
+// they are instead of clearing them.
 //
 // Point records are read two points ahead and the next sub-cell table's
 // taps are loaded before the current point's arithmetic, so both latencies
@@ -1382,19 +1345,6 @@ void build_tables(const T* conv, size_t P, const int* support, int ov, int C,
     }
 }
 
-template <typename T>
-T* dmalloc(size_t count)
-{
-    void* p = nullptr;
-    if (count == 0) count = 1;
-    check_hip(hipMalloc(&p, count * sizeof(T)), kErrMemAlloc, "hipMalloc");
-    return static_cast<T*>(p);
-}
-
-void dfree(void* p)
-{
-    if (p) (void)hipFree(p);
-}
 
 }  // namespace
 
@@ -2107,6 +2057,44 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
     }
 }
 
+__global__ void k_set_u32(unsigned* p, unsigned v)
+{
+    *p = v;
+}
+
+size_t radix_hist_words(size_t n)
+{
+    return 256 * ((n + kRadixChunk - 1) / kRadixChunk);
+}
+
+int radix_sort_enqueue(size_t n, int bits, unsigned long long* keys[2],
+                       unsigned* vals[2], unsigned* hist, unsigned* offs,
+                       unsigned* part, unsigned* d_len, unsigned* d_total,
+                       hipStream_t s)
+{
+    const unsigned nchunk = (unsigned)((n + kRadixChunk - 1) / kRadixChunk);
+    if (nchunk == 0) return 0;
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, d_len,
+            256u * nchunk);
+    int cur = 0;
+    for (int shift = 0; shift < bits; shift += 8)
+    {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nchunk), dim3(kRadixThreads),
+                0, s, n, (const unsigned long long*)keys[cur], shift, nchunk,
+                hist);
+        scan_u32_enqueue(hist, offs, d_len, 256 * (size_t)nchunk, d_total,
+                part, s);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nchunk),
+                dim3(kRadixThreads), 0, s, n,
+                (const unsigned long long*)keys[cur],
+                (const unsigned*)vals[cur], shift, nchunk,
+                (const unsigned*)offs, keys[cur ^ 1], vals[cur ^ 1]);
+        cur ^= 1;
+    }
+    check_hip(hipGetLastError(), kErrKernelLaunch, "radix sort launch");
+    return cur;
+}
+
 // Tile ranges of the sorted keys: tstart[t] = first point, tend[t] = one
 // past the last (both zero for empty tiles); bsum->nv32 = points gridded.
 __global__ void __launch_bounds__(256) k_tile_bounds(size_t n,
@@ -2422,8 +2410,7 @@ void Batch::prepare()
         d_vq_ = dmalloc<float4>(cap_points_);
         d_tstart_ = dmalloc<unsigned>(cap_tiles_);
         d_tend_ = dmalloc<unsigned>(cap_tiles_);
-        const size_t rh = 256 * ((cap_points_ + kRadixChunk - 1) /
-                                 kRadixChunk);
+        const size_t rh = radix_hist_words(cap_points_);
         d_rhist_ = dmalloc<unsigned>(rh);
         d_roff_ = dmalloc<unsigned>(rh);
         d_icnt_ = dmalloc<unsigned>(cap_tiles_);
@@ -2441,9 +2428,8 @@ void Batch::prepare()
     if (J)
         check_hip(hipMemcpy(d_desc_, desc_.data(), J * sizeof(BatchJob),
                 hipMemcpyHostToDevice), kErrMemCopy, "upload job table");
-    const unsigned counts[2] = {(unsigned)tiles, (unsigned)(256 *
-            ((pts + kRadixChunk - 1) / kRadixChunk))};
-    check_hip(hipMemcpy(d_ntiles_, counts, sizeof(counts),
+    const unsigned nt = (unsigned)tiles;
+    check_hip(hipMemcpy(d_ntiles_, &nt, sizeof(unsigned),
             hipMemcpyHostToDevice), kErrMemCopy, "upload tile count");
 }
 
@@ -2496,26 +2482,12 @@ void Batch::enqueue()
             n_total_, (const unsigned long long*)d_keys_[0],
             (const BatchJob*)d_desc_, lmax_, sentinel, d_keys_[1], d_bsum_);
     // Stable LSD radix sort of (batch key, point index).
-    unsigned long long* kin = d_keys_[1];
-    unsigned long long* kout = d_keys_[0];
-    unsigned* vin = d_idx_[0];
-    unsigned* vout = d_idx_[1];
-    const unsigned nchunk = (unsigned)((n_total_ + kRadixChunk - 1) /
-                                       kRadixChunk);
-    for (int shift = 0; nchunk && shift < radix_bits_; shift += 8)
-    {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nchunk), dim3(kRadixThreads),
-                0, s, n_total_, (const unsigned long long*)kin, shift, nchunk,
-                d_rhist_);
-        scan_u32_enqueue(d_rhist_, d_roff_, d_ntiles_ + 1,
-                256 * (size_t)nchunk, &d_bsum_->n_buckets, d_part_, s);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nchunk),
-                dim3(kRadixThreads), 0, s, n_total_,
-                (const unsigned long long*)kin, (const unsigned*)vin, shift,
-                nchunk, (const unsigned*)d_roff_, kout, vout);
-        std::swap(kin, kout);
-        std::swap(vin, vout);
-    }
+    unsigned long long* kk[2] = {d_keys_[1], d_keys_[0]};
+    unsigned* vv[2] = {d_idx_[0], d_idx_[1]};
+    const int cur = radix_sort_enqueue(n_total_, radix_bits_, kk, vv,
+            d_rhist_, d_roff_, d_part_, d_ntiles_ + 1, &d_bsum_->n_buckets, s);
+    const unsigned long long* kin = kk[cur];
+    const unsigned* vin = vv[cur];
     hipLaunchKernelGGL(k_tile_bounds, dim3(pblocks), dim3(256), 0, s,
             n_total_, (const unsigned long long*)kin, lmax_,
             (unsigned)tiles_total_, d_tstart_, d_tend_, d_bsum_);

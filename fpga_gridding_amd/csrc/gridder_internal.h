@@ -164,7 +164,22 @@ struct Error
 [[noreturn]] void fail(int code, const std::string& detail);
 void check_hip(hipError_t e, int code, const char* what);
 
-// Host-side argument checks of a gridding call (no device access): raises
+// Device allocation that raises kErrMemAlloc (count 0 allocates one
+// element, so the pointer is never null).
+template <typename T>
+T* dmalloc(size_t count)
+{
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    check_hip(hipMalloc(&p, count * sizeof(T)), kErrMemAlloc, "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+inline void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
 // Compensated two-float grid (re_hi, re_lo, im_hi, im_lo per cell) to a
 // float grid (re, im per cell), enqueued on `stream`.
 void resolve_compensated(const float* comp, size_t num_cells, float* out,
@@ -173,6 +188,7 @@ void resolve_compensated(const float* comp, size_t num_cells, float* out,
 // Streaming device-to-device copy (k_copy) on the null stream; waits.
 void device_copy(void* dst, const void* src, size_t bytes);
 
+// Host-side argument checks of a gridding call (no device access): raises
 // kErrInvalidArgument for sizes the gridder cannot handle, including a
 // kernel cube too small for support and oversample.
 void validate_args(size_t num_w_planes, const int* support, int oversample,
@@ -420,6 +436,61 @@ private:
     bool timed_ = false;
     int grid_blocks_ = 0;
     int n_cu_ = 0;
+};
+
+// Stable LSD radix sort of n (key, value) pairs on the low `bits` bits of
+// the keys (8-bit digits, one pass per digit), capture-safe.  keys[0] /
+// vals[0] hold the input, keys[1] / vals[1] are scratch of the same size;
+// returns which pair holds the result.  hist and offs hold
+// radix_hist_words(n) words, part the scan's partial sums
+// (radix_hist_words(n) / 4096 + 2 words), d_len and d_total one word each.
+size_t radix_hist_words(size_t n);
+int radix_sort_enqueue(size_t n, int bits, unsigned long long* keys[2],
+                       unsigned* vals[2], unsigned* hist, unsigned* offs,
+                       unsigned* part, unsigned* d_len, unsigned* d_total,
+                       hipStream_t s);
+
+// Uniform-weighting step (oskar_grid_weights_write / _read) on device
+// pointers, FP32 (kPrecSingle) or FP64 (kPrecDouble); each call runs on `s`
+// (nullptr: the object's own stream), waits, and returns the number of
+// points off the grid.
+class Weights
+{
+public:
+    Weights();
+    ~Weights();
+    void set_stream(hipStream_t s);
+    hipStream_t stream() const { return stream_; }
+    size_t write(int prec, size_t n, const void* uu, const void* vv,
+                 const void* weight, double cell, int G, void* grid,
+                 hipStream_t s = nullptr);
+    size_t read(int prec, size_t n, const void* uu, const void* vv,
+                const void* w_in, void* w_out, double cell, int G,
+                const void* grid, hipStream_t s = nullptr);
+    size_t device_bytes() const;
+
+private:
+    template <typename T>
+    size_t write_t(size_t n, const T* uu, const T* vv, const T* weight,
+                   T cell, int G, T* grid, hipStream_t st);
+    template <typename T>
+    size_t read_t(size_t n, const T* uu, const T* vv, const T* w_in,
+                  T* w_out, T cell, int G, const T* grid, hipStream_t st);
+    void ensure_capacity(size_t n);
+    void free_buffers();
+    size_t finish_skipped(hipStream_t st);
+
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+    int n_cu_ = 0;
+    size_t cap_ = 0;
+    unsigned long long* d_keys_[2] = {nullptr, nullptr};
+    unsigned* d_idx_[2] = {nullptr, nullptr};
+    unsigned* d_hist_ = nullptr;
+    unsigned* d_offs_ = nullptr;
+    unsigned* d_part_ = nullptr;
+    unsigned long long* d_small_ = nullptr;   // skipped; scan length, total
+    unsigned long long* h_skipped_ = nullptr;
 };
 
 // Zero-fill with a kernel (capture-safe, unlike hipMemsetAsync).

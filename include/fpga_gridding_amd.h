@@ -206,6 +206,26 @@ GW_EXPORT void gw_batch_results(gw_batch* batch, int job, size_t* num_skipped,
 GW_EXPORT double gw_batch_last_ms(gw_batch* batch);
 
 /* ---------------------------------------------------------------------- */
+/* Uniform weighting on device pointers                                    */
+/* ---------------------------------------------------------------------- */
+/* oskar_grid_weights_write_* / _read_* (oskar_grid_weights.h in this
+ * include/ directory) on DEVICE pointers, precision GW_SINGLE (float
+ * arrays) or GW_DOUBLE (double arrays); cell_size_rad is converted to the
+ * precision of the call.  Enqueued on hip_stream (NULL: a stream of the
+ * calling thread's cached workspace); returns after completion.  Results
+ * are bit-exact with the reference's loops (utils/oskar_grid_weights.c:
+ * 38-172): grid additions of one cell happen in input order. */
+GW_EXPORT void gw_grid_weights_write_device(int precision, size_t num_points,
+        const void* d_uu, const void* d_vv, const void* d_weight,
+        double cell_size_rad, int grid_size, size_t* num_skipped,
+        void* d_grid, void* hip_stream, int* status);
+GW_EXPORT void gw_grid_weights_read_device(int precision, size_t num_points,
+        const void* d_uu, const void* d_vv, const void* d_weight_in,
+        void* d_weight_out, double cell_size_rad, int grid_size,
+        size_t* num_skipped, const void* d_grid, void* hip_stream,
+        int* status);
+
+/* ---------------------------------------------------------------------- */
 /* Serving: a request loop with admission by device-memory budget         */
 /* ---------------------------------------------------------------------- */
 /* A server owns a worker thread on the current HIP device.  Requests (the

@@ -259,6 +259,28 @@ def grid_weights_write_f(uu, vv, weight, cell_size_rad, grid_size, grid,
     return int(sk.value)
 
 
+def grid_weights_write_d(uu, vv, weight, cell_size_rad, grid_size, grid):
+    """gridref_grid_weights_write_d (utils/oskar_grid_weights.c:38-70):
+    adds into ``grid`` (float64); returns num_skipped."""
+    sk = ct.c_size_t(0)
+    uu, vv, w = _c(uu, np.float64), _c(vv, np.float64), _c(weight, np.float64)
+    lib().gridref_grid_weights_write_d(uu.shape[0], uu, vv, w, cell_size_rad,
+                                       grid_size, ct.byref(sk), grid)
+    return int(sk.value)
+
+
+def grid_weights_read_d(uu, vv, weight_in, weight_out, cell_size_rad,
+                        grid_size, grid):
+    """gridref_grid_weights_read_d (utils/oskar_grid_weights.c:72-104)."""
+    sk = ct.c_size_t(0)
+    uu, vv = _c(uu, np.float64), _c(vv, np.float64)
+    lib().gridref_grid_weights_read_d(uu.shape[0], uu, vv,
+                                      _c(weight_in, np.float64), weight_out,
+                                      cell_size_rad, grid_size, ct.byref(sk),
+                                      grid)
+    return int(sk.value)
+
+
 def grid_weights_read_f(uu, vv, weight_in, weight_out, cell_size_rad,
                         grid_size, grid, lib_=None):
     L = lib_ if lib_ is not None else lib()

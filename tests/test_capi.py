@@ -22,7 +22,8 @@ def _ensure_built():
 
 def _declared_symbols():
     names = set()
-    for h in ("oskar_grid_wproj.h", "fpga_gridding_amd.h"):
+    for h in sorted(f for f in os.listdir(os.path.join(ROOT, "include"))
+                    if f.endswith(".h")):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = "\n".join(l for l in text.splitlines()
                          if not l.lstrip().startswith("#"))
@@ -36,7 +37,9 @@ def test_library_exports_every_declared_symbol():
     _ensure_built()
     declared = _declared_symbols()
     assert {"oskar_grid_wproj_f", "oskar_grid_wproj_d",
-            "oskar_grid_wproj_f_diagnostics"} <= declared
+            "oskar_grid_wproj_f_diagnostics", "oskar_grid_weights_write_f",
+            "oskar_grid_weights_read_f", "oskar_grid_weights_write_d",
+            "oskar_grid_weights_read_d"} <= declared
     lib = ct.CDLL(fg.LIB_PATH)
     for name in sorted(declared):
         assert hasattr(lib, name), name
