@@ -22,6 +22,7 @@ GW_SINGLE = 0x01
 GW_DOUBLE = 0x02
 GW_HALF = 0x03          # FP16 storage, FP32 products, compensated grid
 GW_BFLOAT16 = 0x04      # BF16 storage, FP32 products, compensated grid
+GW_SINGLE_INDEX64 = 0x05  # GW_SINGLE data with FP64 index math
 
 _lib = None
 
@@ -178,7 +179,7 @@ def lib():
     L.gw_batch_results.restype = None
     L.gw_batch_last_ms.argtypes = [_vp]
     L.gw_batch_last_ms.restype = ct.c_double
-    L.gw_server_create.argtypes = [_sz, ct.c_int, ct.c_int, _ip]
+    L.gw_server_create.argtypes = [_sz, ct.c_int, ct.c_int, ct.c_int, _ip]
     L.gw_server_create.restype = _vp
     L.gw_server_free.argtypes = [_vp]
     L.gw_server_free.restype = None
@@ -288,6 +289,31 @@ def grid_workload_reduced(wl, precision, comp=None):
         for x in (*d, d_vis, d_comp, d_out):
             x.close()
         return out, sk, nd, comp_out
+    finally:
+        plan.close()
+
+
+def grid_workload_index64(wl):
+    """Grid a synth.Workload with a GW_SINGLE_INDEX64 plan (FP32 data, FP64
+    index math); cell_size_rad and w_scale passed rounded to float, as the
+    FP64 oracle receives them.  Returns (grid float32 [2G^2], skipped,
+    norm)."""
+    g = wl.grid_size
+    plan = Plan(GW_SINGLE_INDEX64, wl.num_w_planes, wl.support,
+                wl.oversample, wl.conv_size_half, wl.conv_func,
+                float(np.float32(wl.cell_size_rad)),
+                float(np.float32(wl.w_scale)), g)
+    try:
+        d = [DeviceArray.from_host(np.ascontiguousarray(a, np.float32))
+             for a in (wl.uu, wl.vv, wl.ww, wl.vis)]
+        d_grid = DeviceArray(8 * g * g)
+        d_grid.zero()
+        plan.grid(wl.num_points, *d, d_grid)
+        sk, nd, _ = plan.results()
+        out = d_grid.download(np.empty(2 * g * g, np.float32))
+        for x in (*d, d_grid):
+            x.close()
+        return out, sk, nd
     finally:
         plan.close()
 
@@ -673,10 +699,12 @@ class Server:
     arguments of oskar_grid_wproj_f on host arrays; admitted by the device
     memory budget and gridded in batches on several streams."""
 
-    def __init__(self, hbm_budget, max_batch_jobs=64, num_streams=2):
+    def __init__(self, hbm_budget, max_batch_jobs=64, num_streams=2,
+                 precision=None):
         st = ct.c_int(0)
+        prec = GW_SINGLE if precision is None else precision
         self.ptr = lib().gw_server_create(int(hbm_budget), max_batch_jobs,
-                                          num_streams, ct.byref(st))
+                                          num_streams, prec, ct.byref(st))
         _check(st, "gw_server_create")
         self._keep = {}
 

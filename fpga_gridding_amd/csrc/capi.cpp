@@ -964,7 +964,7 @@ GW_EXPORT void gw_grid_weights_read_device(int precision, size_t num_points,
 }
 
 GW_EXPORT gw_server* gw_server_create(size_t hbm_budget_bytes,
-        int max_batch_jobs, int num_streams, int* status)
+        int max_batch_jobs, int num_streams, int precision, int* status)
 {
     int local = 0;
     if (!status) status = &local;
@@ -972,7 +972,7 @@ GW_EXPORT gw_server* gw_server_create(size_t hbm_budget_bytes,
     gw_server* out = nullptr;
     guarded(status, [&] {
         std::unique_ptr<gw::Server> sv(new gw::Server(hbm_budget_bytes,
-                max_batch_jobs, num_streams));
+                max_batch_jobs, num_streams, precision));
         out = new gw_server{sv.release()};
     });
     return out;

@@ -1384,12 +1384,14 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
            int oversample, int conv_size_half, const void* conv_func,
            double cell_size_rad, double w_scale, int grid_size)
 {
-    if (precision < kPrecSingle || precision > kPrecBFloat16)
+    if (precision < kPrecSingle || precision > kPrecSingleIndex64)
         fail(kErrInvalidArgument, "precision must be GW_SINGLE, GW_DOUBLE, "
-             "GW_HALF or GW_BFLOAT16");
+             "GW_HALF, GW_BFLOAT16 or GW_SINGLE_INDEX64");
     validate_args(num_w_planes, support, oversample, conv_size_half,
             conv_func, grid_size);
-    prec_ = precision;
+    index64_ = precision == kPrecSingleIndex64;
+    prec_ = index64_ ? kPrecSingle : precision;
+    precision = prec_;
     P_ = num_w_planes;
     ov_ = oversample;
     C_ = conv_size_half;
@@ -1868,7 +1870,15 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
 void Plan::locate(size_t n, const void* uu, const void* vv, const void* ww,
                   const void* vis, const DiagBuffers* diag)
 {
-    if (prec_ == kPrecSingle)
+    if (prec_ == kPrecSingle && index64_)
+    {
+        if (diag)
+            fail(kErrInvalidArgument, "diagnostics need a GW_SINGLE plan");
+        locate_t<double, float, float, RecF>(n, (const float*)uu,
+                (const float*)vv, (const float*)ww, (const float*)vis,
+                nullptr);
+    }
+    else if (prec_ == kPrecSingle)
         locate_t<float>(n, (const float*)uu, (const float*)vv,
                 (const float*)ww, (const float*)vis, diag);
     else if (prec_ == kPrecDouble)
@@ -2453,17 +2463,33 @@ void Batch::enqueue()
         {
             const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
                     (e.n + 255) / 256, (size_t)n_cu_ * 4));
-            hipLaunchKernelGGL((k_locate<float, false, float, float, RecF>),
-                    dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
-                    e.vis, pl.grid_scale_f(), pl.w_scale_f(), pl.oversample(),
-                    d.G, P, pl.device_support(), d.sub_h, K, d.tile_bits,
-                    pl.combo_bits(), d.key_bits, d_keys_[0] + d.point_base,
-                    d_idx_[0] + d.point_base,
-                    static_cast<RecF*>(d_rec_) + d.point_base, combo,
-                    d_sum_ + j, (unsigned*)nullptr,
-                    (unsigned long long*)nullptr,
-                    (unsigned long long*)nullptr, 0,
-                    (unsigned long long)j << kBatchJobShift, d.point_base);
+            if (pl.index64())
+                hipLaunchKernelGGL(
+                        (k_locate<double, false, float, float, RecF>),
+                        dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
+                        e.vis, pl.grid_scale_d(), pl.w_scale_d(),
+                        pl.oversample(), d.G, P, pl.device_support(), d.sub_h,
+                        K, d.tile_bits, pl.combo_bits(), d.key_bits,
+                        d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
+                        static_cast<RecF*>(d_rec_) + d.point_base, combo,
+                        d_sum_ + j, (unsigned*)nullptr,
+                        (unsigned long long*)nullptr,
+                        (unsigned long long*)nullptr, 0,
+                        (unsigned long long)j << kBatchJobShift,
+                        d.point_base);
+            else
+                hipLaunchKernelGGL((k_locate<float, false, float, float, RecF>),
+                        dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
+                        e.vis, pl.grid_scale_f(), pl.w_scale_f(),
+                        pl.oversample(), d.G, P, pl.device_support(), d.sub_h,
+                        K, d.tile_bits, pl.combo_bits(), d.key_bits,
+                        d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
+                        static_cast<RecF*>(d_rec_) + d.point_base, combo,
+                        d_sum_ + j, (unsigned*)nullptr,
+                        (unsigned long long*)nullptr,
+                        (unsigned long long*)nullptr, 0,
+                        (unsigned long long)j << kBatchJobShift,
+                        d.point_base);
         }
         hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, P, K * K,
                 (const unsigned*)combo, pl.device_pinfo(),

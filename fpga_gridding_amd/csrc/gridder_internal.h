@@ -80,7 +80,9 @@ enum
     kPrecDouble = 0x02,     // FP64 throughout (oskar_grid_wproj_d)
     kPrecHalf = 0x03,       // FP16 taps and visibilities, FP64 index math,
                             // FP32 products, compensated two-float grid
-    kPrecBFloat16 = 0x04    // as kPrecHalf with BF16 storage
+    kPrecBFloat16 = 0x04,   // as kPrecHalf with BF16 storage
+    kPrecSingleIndex64 = 0x05   // kPrecSingle with FP64 index math (the
+                                // plan's precision() stays kPrecSingle)
 };
 
 // 16-bit storage types.  FP16 is the compiler's _Float16; BF16 is the top
@@ -260,6 +262,10 @@ public:
     int sub_tables() const { return K_; }
     float grid_scale_f() const { return gscale_f_; }
     float w_scale_f() const { return ws_f_; }
+    double grid_scale_d() const { return gscale_d_; }
+    double w_scale_d() const { return ws_d_; }
+    // FP64 index math on FP32 data (created as kPrecSingleIndex64).
+    bool index64() const { return index64_; }
     int grid_blocks() const { return grid_blocks_; }
     // Device bytes of the plan's constant tables (kernel set).
     size_t table_bytes() const { return table_bytes_; }
@@ -278,6 +284,7 @@ private:
                   size_t max_n, unsigned* total);
 
     int prec_;
+    bool index64_ = false;
     size_t P_;
     int ov_, C_, G_, h_, K_;
     int s_max_;

@@ -35,11 +35,16 @@ void* dalloc(size_t bytes)
 
 }  // namespace
 
-Server::Server(size_t hbm_budget, int max_batch_jobs, int n_streams)
+Server::Server(size_t hbm_budget, int max_batch_jobs, int n_streams,
+               int precision)
     : budget_(hbm_budget),
+      precision_(precision),
       max_batch_(std::max(1, max_batch_jobs)),
       slots_(std::max(1, std::min(n_streams, 8)))
 {
+    if (precision != kPrecSingle && precision != kPrecSingleIndex64)
+        fail(kErrInvalidArgument, "the server grids GW_SINGLE or "
+             "GW_SINGLE_INDEX64 requests");
     check_hip(hipGetDevice(&device_), kErrComputeDevices, "hipGetDevice");
     for (int i = 0; i < slots_; ++i)
         {
@@ -214,7 +219,7 @@ Plan* Server::plan_for(const Request& r, size_t* new_bytes)
         return it->second.plan.get();
     }
     const KernelSet& k = ksets_[r.kernel_set];
-    std::unique_ptr<Plan> p(new Plan(kPrecSingle, k.P, k.support.data(),
+    std::unique_ptr<Plan> p(new Plan(precision_, k.P, k.support.data(),
             k.ov, k.C, k.cube.data(), r.cell_size_rad, r.w_scale,
             r.grid_size));
     *new_bytes = p->table_bytes();

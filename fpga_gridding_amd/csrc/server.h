@@ -53,8 +53,10 @@ public:
 
     // hbm_budget: device bytes the server may hold (tables, grids, inputs,
     // workspace); max_batch_jobs: most requests per batch; n_streams:
-    // batches in flight at once.
-    Server(size_t hbm_budget, int max_batch_jobs, int n_streams);
+    // batches in flight at once; precision: kPrecSingle (the index math of
+    // oskar_grid_wproj_f) or kPrecSingleIndex64 (FP64 index math).
+    Server(size_t hbm_budget, int max_batch_jobs, int n_streams,
+           int precision = kPrecSingle);
     ~Server();
     int add_kernel_set(size_t num_w_planes, const int* support,
                        int oversample, int conv_size_half,
@@ -126,6 +128,7 @@ private:
     }
 
     size_t budget_;
+    int precision_;
     int max_batch_;
     int slots_;
     int device_ = 0;

@@ -49,7 +49,14 @@ enum gw_status_code
  * oskar_grid_wproj_d), FP32 products, FP64 per-job sums and a compensated
  * two-float grid; the norm is summed in FP64 over the stored taps. */
 enum gw_precision { GW_SINGLE = 0x01, GW_DOUBLE = 0x02, GW_HALF = 0x03,
-                    GW_BFLOAT16 = 0x04 };
+                    GW_BFLOAT16 = 0x04, GW_SINGLE_INDEX64 = 0x05 };
+/* GW_SINGLE_INDEX64 is GW_SINGLE (FP32 inputs, taps, products and grid;
+ * the same data layout, batches and server) with the index math of
+ * oskar_grid_wproj_d: plane, cell and sub-cell offset computed in FP64 from
+ * the float coordinates, so that no FP32 rounding flips an index.  Its
+ * results then differ from the FP64 oracle only by FP32 arithmetic (and
+ * its num_skipped is oskar_grid_wproj_d's), where GW_SINGLE keeps
+ * oskar_grid_wproj_f's FP32 index semantics. */
 
 /* ---------------------------------------------------------------------- */
 /* Status-returning drop-in calls                                          */
@@ -118,7 +125,8 @@ GW_EXPORT void gw_plan_set_stream(gw_plan* plan, void* hip_stream);
  * reference.  Asynchronous on the plan's stream: read num_skipped and the
  * norm increment with gw_plan_results().  Element types by precision:
  *   GW_SINGLE   uu/vv/ww float, d_vis 2*num_points float,
- *               d_grid 2*grid_size^2 float (re, im);
+ *               d_grid 2*grid_size^2 float (re, im); GW_SINGLE_INDEX64
+ *               likewise;
  *   GW_DOUBLE   the same with double;
  *   GW_HALF     uu/vv/ww float, d_vis 2*num_points FP16 (_Float16),
  *               d_grid 4*grid_size^2 float: per cell re_hi, re_lo, im_hi,
@@ -258,9 +266,11 @@ typedef struct gw_server_stats
 } gw_server_stats;
 
 /* hbm_budget_bytes: device memory the server may hold; max_batch_jobs:
- * most requests gridded together; num_streams: batches in flight (1-8). */
+ * most requests gridded together; num_streams: batches in flight (1-8);
+ * precision: GW_SINGLE (oskar_grid_wproj_f's index math) or
+ * GW_SINGLE_INDEX64 (FP64 index math), for every request. */
 GW_EXPORT gw_server* gw_server_create(size_t hbm_budget_bytes,
-        int max_batch_jobs, int num_streams, int* status);
+        int max_batch_jobs, int num_streams, int precision, int* status);
 /* Finishes every queued request, then frees all device memory. */
 GW_EXPORT void gw_server_free(gw_server* server);
 /* Registers a W-kernel set (arguments as oskar_grid_wproj_f's, copied);

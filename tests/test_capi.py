@@ -130,7 +130,7 @@ def test_no_device_reports_compute_device_error():
     assert e.value.status == -19
 
 
-@pytest.mark.parametrize("precision", [0, 5, -1])
+@pytest.mark.parametrize("precision", [0, 6, -1])
 def test_plan_rejects_unknown_precision(precision):
     """gw_plan_create checks the precision code before any GPU call."""
     _ensure_built()
