@@ -80,6 +80,9 @@ def accuracy_sample(wl, n_sample=200_000, seed=4):
     ref_d = oracle.grid_wproj_d(sub)[0]
     out["fp64_oracle"] = err(g, ref_d)
     out["exact_oracle"] = err(g, oracle.grid_wproj_f_exact(sub)[0])
+    # FP32 data with FP64 index math (the FP64 oracle's cells).
+    out["fp32_index64_vs_fp64_oracle"] = err(
+        fg.grid_workload_index64(sub)[0], ref_d)
     # Reduced-precision plans (FP64 index math, so the FP64 oracle's cells).
     for prec, name in ((fg.GW_HALF, "fp16"), (fg.GW_BFLOAT16, "bf16")):
         out[f"{name}_vs_fp64_oracle"] = err(
@@ -148,12 +151,35 @@ def cpu_baseline(wl, budget_s=12.0, seed=3):
                 updates_per_s=done_upd / elapsed, seconds=elapsed)
 
 
-def precision_variants(wl, d_uvw, args):
-    """Mvis/s of the FP16 and BF16 plans on the same resident points (16-bit
-    visibilities, compensated two-float grid), same steps as the headline."""
+def precision_variants(wl, d_uvw, d_vis32, d_grid32, args):
+    """Mvis/s of the FP32 plan with FP64 index math and of the FP16 and BF16
+    plans (16-bit visibilities, compensated two-float grid) on the same
+    resident points, same steps as the headline."""
     import fpga_gridding_amd as fg
     out = {}
     G = wl.grid_size
+    plan = fg.Plan(fg.GW_SINGLE_INDEX64, wl.num_w_planes, wl.support,
+                   wl.oversample, wl.conv_size_half, wl.conv_func,
+                   float(np.float32(wl.cell_size_rad)),
+                   float(np.float32(wl.w_scale)), G)
+    for _ in range(args.warmup):
+        plan.grid(wl.num_points, *d_uvw, d_vis32, d_grid32)
+        plan.results()
+    fg.device_synchronize()
+    stage = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.grid(wl.num_points, *d_uvw, d_vis32, d_grid32)
+        plan.results()
+        stage.append(plan.timings())
+    fg.device_synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    out["fp32_index64"] = {"value": wl.num_points / (ms * 1e-3) / 1e6,
+                           "unit": "Mvis/s", "ms_per_step": ms,
+                           "scatter_ms": float(np.mean([x[3] for x in stage])),
+                           "storage": "fp32 taps, visibilities and grid, FP64 "
+                                      "index math"}
+    plan.close()
     for prec, name in ((fg.GW_HALF, "fp16"), (fg.GW_BFLOAT16, "bf16")):
         plan = fg.Plan(prec, wl.num_w_planes, wl.support, wl.oversample,
                        wl.conv_size_half, wl.conv_func,
@@ -191,7 +217,7 @@ MIX_WORKLOAD = ("64 jobs from the 512-station workload: random subsets of "
                 "w-planes, support 4..4-36, oversample 4 (seed 7)")
 
 
-def _mix_setup(autotune=True):
+def _mix_setup(precision, autotune=True):
     """The 64-job mix on the device: plans (one per kernel set and grid
     side, autotuned on their largest job), points and one grid per job."""
     import fpga_gridding_amd as fg
@@ -202,7 +228,7 @@ def _mix_setup(autotune=True):
     for wl in jobs:
         key = (wl.params["kernel_set"], wl.grid_size)
         if key not in plans:
-            plans[key] = fg.Plan.for_workload(wl)
+            plans[key] = fg.Plan.for_workload(wl, precision)
     dev = []
     for wl in jobs:
         d = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww,
@@ -227,13 +253,30 @@ def _mix_setup(autotune=True):
     return jobs, sets, plans, dev, tune_s
 
 
+def _vs_fp64_oracle(jobs, grids):
+    """max|d|/max|g| and RMS(d)/RMS(g) of each grid against the FP64 oracle
+    on its job (worst over the jobs)."""
+    import oracle
+    mx, rms = 0.0, 0.0
+    for wl, g in zip(jobs, grids):
+        ref = oracle.grid_wproj_d(wl)[0]
+        d = g.astype(np.float64) - ref
+        if np.any(ref):
+            mx = max(mx, float(np.abs(d).max() / np.abs(ref).max()))
+            rms = max(rms, float(np.sqrt(np.mean(d * d)) /
+                                 np.sqrt(np.mean(ref * ref))))
+    return {"jobs": len(jobs), "fp64_oracle": {"max_rel": mx,
+                                                "rms_rel": rms}}
+
+
 def bench_batched(args):
     """Batched jobs: the 64-job mix in one pipeline with one persistent
     scatter launch, replayed from a HIP graph, against 64 sequential plain
     per-job calls (gw_grid_device + gw_plan_results each) on the same plans
     and resident data."""
     import fpga_gridding_amd as fg
-    jobs, sets, plans, dev, tune_s = _mix_setup()
+    prec = fg.GW_SINGLE if args.index32 else fg.GW_SINGLE_INDEX64
+    jobs, sets, plans, dev, tune_s = _mix_setup(prec)
     key = [(wl.params["kernel_set"], wl.grid_size) for wl in jobs]
     n_total = sum(wl.num_points for wl in jobs)
 
@@ -267,6 +310,7 @@ def bench_batched(args):
         sk, nd, nu = b.results(j)
         agree &= (sk, nu) == (ref[j][0], ref[j][2])
         agree &= abs(nd - ref[j][1]) <= 1e-12 * max(abs(ref[j][1]), 1.0)
+    acc = {}
     for j in small:
         gj = dev[j][1].download(np.empty(2 * jobs[j].grid_size ** 2,
                                          np.float32))
@@ -274,6 +318,9 @@ def bench_batched(args):
         if s > 0:
             worst = max(worst, float(np.abs(gj.astype(np.float64) -
                                             ref_g[j]).max() / s))
+        acc[j] = gj
+    accuracy = _vs_fp64_oracle([jobs[j] for j in small],
+                               [acc[j] for j in small])
 
     for _ in range(args.warmup):
         sequential()
@@ -315,6 +362,9 @@ def bench_batched(args):
         "autotune_s": tune_s,
         "check": {"results_identical": bool(agree),
                   "grid_max_rel_4_smallest": worst},
+        "precision": "fp32_index64" if prec == fg.GW_SINGLE_INDEX64 else
+                     "fp32",
+        "accuracy": accuracy,
     }
     b.close()
     return out
@@ -328,7 +378,8 @@ def bench_served(args):
     arrivals."""
     import threading
     import fpga_gridding_amd as fg
-    jobs, sets, plans, dev, tune_s = _mix_setup(autotune=False)
+    prec = fg.GW_SINGLE if args.index32 else fg.GW_SINGLE_INDEX64
+    jobs, sets, plans, dev, tune_s = _mix_setup(prec, autotune=False)
     rng = np.random.default_rng(11)
     R = args.requests
     order = rng.integers(0, len(jobs), R)
@@ -365,7 +416,7 @@ def bench_served(args):
         p.close()
 
     sv = fg.Server(args.hbm_budget_gib << 30, max_batch_jobs=16,
-                   num_streams=2)
+                   num_streams=2, precision=prec)
     kid = [sv.add_kernel_set(k.num_w_planes, k.support, k.oversample,
                              k.conv_size_half, k.conv_func) for k in sets]
     gids = {}
@@ -407,6 +458,16 @@ def bench_served(args):
     first = min(x.t_submit for x in res)
     last = max(x.t_done for x in res)
     st = sv.stats()
+    # Accuracy: the four smallest jobs once more, each into a fresh grid.
+    small = sorted(range(len(jobs)), key=lambda j: jobs[j].num_points)[:4]
+    outs = []
+    for j in small:
+        wl = jobs[j]
+        rid = sv.submit(kid[wl.params["kernel_set"]], wl.grid_size,
+                        wl.cell_size_rad, wl.w_scale, wl.uu, wl.vv, wl.ww,
+                        wl.vis)
+        outs.append(sv.wait(rid)[1])
+    accuracy = _vs_fp64_oracle([jobs[j] for j in small], outs)
     sv.close()
     lp = np.array(lat_plain)
     return {
@@ -428,6 +489,9 @@ def bench_served(args):
         "offered_mvis_s": n_total / arrive[-1] / 1e6,
         "served_span_s": served_span,
         "server_stats": st,
+        "precision": "fp32_index64" if prec == fg.GW_SINGLE_INDEX64 else
+                     "fp32",
+        "accuracy": accuracy,
         "plain_loop": {"value": n_total / plain_span / 1e6,
                        "unit": "Mvis/s",
                        "latency_ms": {"p50": float(np.percentile(lp, 50)
@@ -455,6 +519,9 @@ def main():
                     help="served: request arrival rate (requests/s)")
     ap.add_argument("--requests", type=int, default=400,
                     help="served: number of requests")
+    ap.add_argument("--index32", action="store_true",
+                    help="batched / served: FP32 index math (GW_SINGLE) "
+                         "instead of FP64 (GW_SINGLE_INDEX64)")
     ap.add_argument("--hbm-budget-gib", type=int, default=64,
                     help="served: device-memory budget of the server")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -558,7 +625,8 @@ def main():
                      "flops_per_launch": flops},
     }
     if not args.no_variants:
-        out["precision_variants"] = precision_variants(wl, d_uvw, args)
+        out["precision_variants"] = precision_variants(wl, d_uvw, d_vis,
+                                                       d_grid, args)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         out["cpu_baseline"] = {k: cb[k] for k in
