@@ -151,6 +151,22 @@ def cpu_baseline(wl, budget_s=12.0, seed=3):
                 updates_per_s=done_upd / elapsed, seconds=elapsed)
 
 
+def dropin_rate(wl, calls=2):
+    """Mvis/s of the drop-in oskar_grid_wproj_f on host arrays (uploads of
+    the points, read-modify-write of the touched grid rows over PCIe)."""
+    import fpga_gridding_amd as fg
+    grid = np.zeros(2 * wl.grid_size ** 2, np.float32)
+    fg.grid_workload_f(wl, grid=grid)           # warm-up (tables uploaded)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        fg.grid_workload_f(wl, grid=grid)
+    ms = (time.perf_counter() - t0) / calls * 1e3
+    return {"value": wl.num_points / (ms * 1e-3) / 1e6, "unit": "Mvis/s",
+            "ms_per_call": ms,
+            "what": "oskar_grid_wproj_f on host arrays, grid accumulated "
+                    "in host memory"}
+
+
 def precision_variants(wl, d_uvw, d_vis32, d_grid32, args):
     """Mvis/s of the FP32 plan with FP64 index math and of the FP16 and BF16
     plans (16-bit visibilities, compensated two-float grid) on the same
@@ -627,6 +643,7 @@ def main():
     if not args.no_variants:
         out["precision_variants"] = precision_variants(wl, d_uvw, d_vis,
                                                        d_grid, args)
+        out["dropin_pcie_inclusive"] = dropin_rate(wl)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         out["cpu_baseline"] = {k: cb[k] for k in
