@@ -228,6 +228,7 @@ __global__ void __launch_bounds__(256) k_locate(
         size_t n, const TU* __restrict__ uu, const TU* __restrict__ vv,
         const TU* __restrict__ ww, const TV* __restrict__ vis, T grid_scale,
         T w_scale, int ov, int G, int P, const int* __restrict__ support,
+        const int* __restrict__ srank, int srank_bits,
         int h, int K, int tile_bits, int combo_bits, int key_bits,
         unsigned long long* __restrict__ keys, unsigned* __restrict__ idx,
         Rec* __restrict__ rec,
@@ -248,14 +249,16 @@ __global__ void __launch_bounds__(256) k_locate(
 
     const int centre = G / 2;
     const T ovf = (T)ov;
-    // Sort key, most significant first: tile (Z-order), plane, position
-    // inside the tile (Z-order), sub-cell table.  Skipped points sort last.
-    // (Points at one position then share the scatter's running sums even
-    // when their sub-cell tables differ.)
+    // Sort key, most significant first: tile (Z-order), support rank,
+    // position inside the tile (Z-order), plane, sub-cell table.  Skipped
+    // points sort last.  Points of one support share the footprint shape,
+    // so the scatter takes them as one segment, and points at one position
+    // share its running sums across planes and sub-cell tables.
     const unsigned long long skip_key = 1ull << key_bits;
-    const int sh_combo = 2 * tile_bits;
-    const int sh_plane = sh_combo + combo_bits;
-    const int sh_tile = sh_plane + bits_plane_(P);
+    const int sh_plane = combo_bits;
+    const int sh_pos = sh_plane + bits_plane_(P);
+    const int sh_srank = sh_pos + 2 * tile_bits;
+    const int sh_tile = sh_srank + srank_bits;
     int nskip = 0;
     int vmin = INT_MAX, vmax = INT_MIN, umin = INT_MAX, umax = INT_MIN;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -302,9 +305,10 @@ __global__ void __launch_bounds__(256) k_locate(
             const int c = plane * K * K + table;
             const unsigned long long mort = spread_bits16((unsigned)u0) |
                     (spread_bits16((unsigned)v0) << 1);
-            key = ((mort >> sh_combo) << sh_tile) |
+            key = ((mort >> (2 * tile_bits)) << sh_tile) |
+                    ((unsigned long long)srank[plane] << sh_srank) |
+                    ((mort & ((1ull << (2 * tile_bits)) - 1ull)) << sh_pos) |
                     ((unsigned long long)plane << sh_plane) |
-                    ((mort & ((1ull << sh_combo) - 1ull)) << combo_bits) |
                     (unsigned long long)table;
             if (lds_hist) atomicAdd(&s_combo[c], 1u);
             else atomicAdd(&combo[c], 1u);
@@ -541,9 +545,9 @@ __global__ void __launch_bounds__(256) k_gather(
         const unsigned long long* __restrict__ keys,
         const unsigned* __restrict__ idx,
         const typename Types<T>::Rec* __restrict__ rec,
-        const unsigned* __restrict__ gw, int h, int K, int tile_shift,
-        int4* __restrict__ meta, typename Types<T>::T4* __restrict__ vq,
-        unsigned* __restrict__ head)
+        const unsigned* __restrict__ gw, const PlaneInfo* __restrict__ pinfo,
+        int h, int K, int tile_shift, int4* __restrict__ meta,
+        typename Types<T>::T4* __restrict__ vq, unsigned* __restrict__ head)
 {
     const unsigned nv = summary->nv32;
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
@@ -559,7 +563,10 @@ __global__ void __launch_bounds__(256) k_gather(
         const int off_u = (r.sub & 0xFF) - 128;
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
         const int table = (off_v + h) * K + (off_u + h);
-        meta[s] = make_int4(r.pos, table, plane,
+        const PlaneInfo& pi = pinfo[plane];
+        const int W = pi.W;
+        meta[s] = make_int4(r.pos, (int)(pi.eoff + (unsigned)(table * W * W)),
+                plane | (pi.srank << 16),
                 gw ? (int)min(gw[i], 0x7FFFFFFFu) : 0);
         typename Types<T>::T4 q;
         q.x = r.re;
@@ -712,8 +719,8 @@ __device__ __forceinline__ PlaneInfo ld_pinfo(const PlaneInfo* p, int i)
     r.nslot = q[3];
     r.wmagic = (unsigned)q[4];
     r.sshift = q[5];
-    r.eoff = (unsigned long long)(unsigned)q[6] |
-            ((unsigned long long)(unsigned)q[7] << 32);
+    r.eoff = (unsigned)q[6];
+    r.srank = q[7];
     return r;
 }
 
@@ -777,9 +784,8 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
     const int lane = threadIdx.x & 63;
     const int W = pi.W, W2 = W * W;
     const int stride = 64 * pi.wpp;
-    const char* __restrict__ ptab = reinterpret_cast<const char*>(
-            static_cast<const TapT*>(x.etab) + pi.eoff);
-    const size_t table_bytes = (size_t)W2 * sizeof(TapT);
+    // meta.y: first tap of the point's (plane, sub-cell) table.
+    const char* __restrict__ ptab = static_cast<const char*>(x.etab);
     const int cstride = LDS ? x.R : x.G;
     unsigned toff[NS];     // byte offset of the lane's tap in a table
     unsigned loff[NS];     // LDS: byte offset from the footprint origin in
@@ -819,7 +825,7 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
         }
     }
     auto load_taps = [&](int table) {
-        const char* tp = ptab + (size_t)table * table_bytes;
+        const char* tp = ptab + (size_t)(unsigned)table * sizeof(TapT);
 #pragma unroll
         for (int s = 0; s < NS; ++s)
         {
@@ -1071,7 +1077,9 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
     x.vy0 = (int)((unsigned)job.x >> 16);
     const int start = job.y;
     const int len = job.z;
-    // Plane segments of the job (points are sorted by plane).
+    // Support segments of the job (points are sorted by support rank,
+    // meta.z >> 16; meta.z & 0xFFFF is the plane, whose footprint shape all
+    // planes of the segment share).
     unsigned heads = 0;
     int myplane[2] = {0, 0};
     bool myhead[2] = {false, false};
@@ -1081,8 +1089,10 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
         const int i = 2 * tid + q;
         if (i < len)
         {
-            myplane[q] = meta[start + i].z;
-            myhead[q] = (i == 0) || meta[start + i - 1].z != myplane[q];
+            const int z = meta[start + i].z;
+            myplane[q] = z & 0xFFFF;
+            myhead[q] = (i == 0) ||
+                    (meta[start + i - 1].z >> 16) != (z >> 16);
             heads += myhead[q] ? 1u : 0u;
         }
     }
@@ -1306,7 +1316,7 @@ void build_tables(const T* conv, size_t P, const int* support, int ov, int C,
     for (size_t p = 0; p < P; ++p)
     {
         const size_t W = 2 * (size_t)support[p] + 1;
-        pinfo[p].eoff = total;
+        pinfo[p].eoff = (unsigned)total;
         total += (size_t)K * K * W * W;
     }
     if (total * 2 * sizeof(T) > (size_t)8 << 30)
@@ -1411,7 +1421,17 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     combo_bits_ = std::max(1, bits_for((unsigned long long)(K_ * K_ - 1)));
     plane_bits_ = bits_plane_((int)P_);
 
-    key_bits_ = 2 * coord_bits_ + combo_bits_ + plane_bits_;
+    // Support ranks (distinct S values in increasing order).
+    std::vector<int> svals(support_);
+    std::sort(svals.begin(), svals.end());
+    svals.erase(std::unique(svals.begin(), svals.end()), svals.end());
+    srank_.resize(P_);
+    for (size_t p = 0; p < P_; ++p)
+        srank_[p] = (int)(std::lower_bound(svals.begin(), svals.end(),
+                support_[p]) - svals.begin());
+    srank_bits_ = std::max(1, bits_for((unsigned long long)(svals.size() -
+            1)));
+    key_bits_ = 2 * coord_bits_ + combo_bits_ + plane_bits_ + srank_bits_;
     pinfo_.resize(P_);
     for (size_t p = 0; p < P_; ++p)
     {
@@ -1419,6 +1439,7 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
         pi.S = support_[p];
         pi.W = 2 * pi.S + 1;
         pi.eoff = 0;
+        pi.srank = srank_[p];
     }
     configure(-1, 0, 0);
 
@@ -1502,6 +1523,7 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     table_bytes_ = etab_bytes + P_ * (sizeof(int) + sizeof(PlaneInfo)) +
             normtab.size() * sizeof(double);
     d_support_ = dmalloc<int>(P_);
+    d_srank_ = dmalloc<int>(P_);
     d_pinfo_ = dmalloc<PlaneInfo>(P_);
     d_normtab_ = dmalloc<double>(normtab.size());
     d_combo_ = dmalloc<unsigned>(P_ * K_ * K_);
@@ -1513,6 +1535,8 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
             hipMemcpyHostToDevice), kErrMemCopy, "upload tap tables");
     check_hip(hipMemcpy(d_support_, support_.data(), P_ * sizeof(int),
             hipMemcpyHostToDevice), kErrMemCopy, "upload support");
+    check_hip(hipMemcpy(d_srank_, srank_.data(), P_ * sizeof(int),
+            hipMemcpyHostToDevice), kErrMemCopy, "upload support ranks");
     check_hip(hipMemcpy(d_pinfo_, pinfo_.data(), P_ * sizeof(PlaneInfo),
             hipMemcpyHostToDevice), kErrMemCopy, "upload plane table");
     check_hip(hipMemcpy(d_normtab_, normtab.data(),
@@ -1561,7 +1585,7 @@ void Plan::configure(int tile_bits, int slot_cap, int min_share)
         const long long R = (1ll << tile_bits) + wmax - 1;
         region_ = R * R + kSinkCells <= kRegionCells ? (int)R : 0;
     }
-    tile_shift_ = 2 * tile_bits_ + combo_bits_ + plane_bits_;
+    tile_shift_ = 2 * tile_bits_ + combo_bits_ + plane_bits_ + srank_bits_;
     slot_cap_ = slot_cap > 0 ? slot_cap : smax_all;
     min_share_ = min_share > 0 ? min_share : kMinShare;
 
@@ -1651,6 +1675,7 @@ Plan::~Plan()
     free_workspace();
     dfree(d_etab_);
     dfree(d_support_);
+    dfree(d_srank_);
     dfree(d_pinfo_);
     dfree(d_normtab_);
     dfree(d_combo_);
@@ -1788,14 +1813,16 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
     if (diag)
         hipLaunchKernelGGL((k_locate<T, true, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
-                d_support_, h_, K_, tile_bits_, combo_bits_, key_bits_,
+                d_support_, (const int*)d_srank_, srank_bits_, h_, K_,
+                tile_bits_, combo_bits_, key_bits_,
                 d_keys_[0], d_idx_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, d_gw_, diag->per_support,
                 diag->per_plane, diag->plane_cap, 0ull, 0u);
     else
         hipLaunchKernelGGL((k_locate<T, false, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
-                d_support_, h_, K_, tile_bits_, combo_bits_, key_bits_,
+                d_support_, (const int*)d_srank_, srank_bits_, h_, K_,
+                tile_bits_, combo_bits_, key_bits_,
                 d_keys_[0], d_idx_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, (unsigned*)nullptr,
                 (unsigned long long*)nullptr, (unsigned long long*)nullptr,
@@ -1827,8 +1854,9 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     hipLaunchKernelGGL((k_gather<T>), dim3(blocks), dim3(256), 0, stream_,
             (const Summary*)d_summary_,
             (const unsigned long long*)d_keys_[1], (const unsigned*)d_idx_[1],
-            (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr), h_,
-            K_, tile_shift_, d_meta_, (T4*)d_vq_, d_head_);
+            (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
+            (const PlaneInfo*)d_pinfo_, h_, K_, tile_shift_, d_meta_,
+            (T4*)d_vq_, d_head_);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_gather launch");
     // Tiles: exclusive scan of the heads, then their starts.
     scan_u32(d_head_, d_bpos_, &d_summary_->nv32, n, &d_summary_->n_buckets);
@@ -1952,7 +1980,8 @@ constexpr int kBatchJobShift = 48;
 static_assert(offsetof(BatchJob, etab) == 8 && offsetof(BatchJob, grid) == 16
         && offsetof(BatchJob, G) == 24 && offsetof(BatchJob, R) == 28 &&
         offsetof(BatchJob, min_share) == 44, "BatchJob word layout");
-static_assert(sizeof(PlaneInfo) == 32 && offsetof(PlaneInfo, eoff) == 24,
+static_assert(sizeof(PlaneInfo) == 32 && offsetof(PlaneInfo, eoff) == 24 &&
+        offsetof(PlaneInfo, srank) == 28,
         "PlaneInfo word layout");
 constexpr unsigned long long kBatchLocalMask = (1ull << kBatchJobShift) - 1;
 
@@ -2197,7 +2226,10 @@ __global__ void __launch_bounds__(256) k_gather_batch(
         const int off_u = (r.sub & 0xFF) - 128;
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
         const int table = (off_v + d.sub_h) * d.sub_k + (off_u + d.sub_h);
-        meta[s] = make_int4(r.pos, table, plane, 0);
+        const PlaneInfo& pi = d.pinfo[plane];
+        const int W = pi.W;
+        meta[s] = make_int4(r.pos, (int)(pi.eoff + (unsigned)(table * W * W)),
+                plane | (pi.srank << 16), 0);
         vq[s] = make_float4(r.re, -sgn * r.im, r.im, sgn * r.re);
     }
 }
@@ -2468,7 +2500,8 @@ void Batch::enqueue()
                         (k_locate<double, false, float, float, RecF>),
                         dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
                         e.vis, pl.grid_scale_d(), pl.w_scale_d(),
-                        pl.oversample(), d.G, P, pl.device_support(), d.sub_h,
+                        pl.oversample(), d.G, P, pl.device_support(),
+                        pl.device_srank(), pl.srank_bits(), d.sub_h,
                         K, d.tile_bits, pl.combo_bits(), d.key_bits,
                         d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
                         static_cast<RecF*>(d_rec_) + d.point_base, combo,
@@ -2481,7 +2514,8 @@ void Batch::enqueue()
                 hipLaunchKernelGGL((k_locate<float, false, float, float, RecF>),
                         dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
                         e.vis, pl.grid_scale_f(), pl.w_scale_f(),
-                        pl.oversample(), d.G, P, pl.device_support(), d.sub_h,
+                        pl.oversample(), d.G, P, pl.device_support(),
+                        pl.device_srank(), pl.srank_bits(), d.sub_h,
                         K, d.tile_bits, pl.combo_bits(), d.key_bits,
                         d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
                         static_cast<RecF*>(d_rec_) + d.point_base, combo,

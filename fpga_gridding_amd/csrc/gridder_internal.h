@@ -49,8 +49,12 @@ struct PlaneInfo
     unsigned wmagic;
     int sshift;  // log2 of the streams per workgroup, kGridThreads/64/wpp
     // First tap of this plane in the expanded tap table: K*K sub-cell
-    // tables of W*W taps each, tap (j, k) at [(iv*K + iu)*W*W + j*W + k].
-    unsigned long long eoff;
+    // tables of W*W taps each, tap (j, k) at [(iv*K + iu)*W*W + j*W + k]
+    // (tables are limited to 8 GiB, so the offset fits 32 bits).
+    unsigned eoff;
+    // Rank of S among the distinct supports: planes of one rank share the
+    // footprint shape, and the scatter takes their points as one segment.
+    int srank;
 };
 
 // Device-side per-call summary (written by the kernels, read back once).
@@ -253,6 +257,8 @@ public:
     const void* device_etab() const { return d_etab_; }
     const double* device_normtab() const { return d_normtab_; }
     const int* device_support() const { return d_support_; }
+    const int* device_srank() const { return d_srank_; }
+    int srank_bits() const { return srank_bits_; }
     int coord_bits() const { return coord_bits_; }
     int combo_bits() const { return combo_bits_; }
     int tile_shift() const { return tile_shift_; }
@@ -293,11 +299,13 @@ private:
     int coord_bits_;     // bits of a grid coordinate
     int combo_bits_;     // bits of a sub-cell table index
     int plane_bits_;     // bits of a plane index
+    int srank_bits_;     // bits of a support rank
     int tile_bits_;      // log2 of the tile side T
     int tile_shift_;     // key bits below the tile index
     int key_bits_;       // valid keys are < 2^key_bits_
     int region_;         // LDS region side T + Wmax - 1 (0: none)
     std::vector<int> support_;
+    std::vector<int> srank_;
     std::vector<PlaneInfo> pinfo_;
     int min_share_ = kMinShare; // least points per scatter stream
     int slot_cap_ = 8;          // most footprint cells per lane
@@ -312,6 +320,7 @@ private:
     size_t table_bytes_ = 0;
     size_t etab_taps_ = 0;
     int* d_support_ = nullptr;
+    int* d_srank_ = nullptr;
     PlaneInfo* d_pinfo_ = nullptr;
     double* d_normtab_ = nullptr;
 
