@@ -631,6 +631,14 @@ def main():
                           "gather_plan": float(stage[:, 2].mean()),
                           "scatter": grid_ms,
                           "total": float(stage[:, 4].mean())},
+        "hbm_counters": None if traffic is None else {
+            "bytes_per_launch": traffic,
+            "achieved_gbs": traffic / (grid_ms * 1e-3) / 1e9,
+            "peak_gbs": HBM_PEAK_GBS,
+            "frac": traffic / (grid_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of k_grid<float>, "
+                      "copy-calibrated (profiles/k_grid_ska512_traffic.txt), "
+                      "over this run's scatter time"},
         "roofline": {"bound": "mfma", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,

@@ -1319,9 +1319,9 @@ void build_tables(const T* conv, size_t P, const int* support, int ov, int C,
         pinfo[p].eoff = (unsigned)total;
         total += (size_t)K * K * W * W;
     }
-    if (total * 2 * sizeof(T) > (size_t)8 << 30)
+    if (total * 2 * sizeof(T) >= (size_t)4 << 30)
         fail(kErrInvalidArgument,
-             "support and oversample need more than 8 GiB of tap tables");
+             "support and oversample need 4 GiB or more of tap tables");
     etab.resize(2 * total);
     normtab.assign(P * K * K, 0.0);
     const size_t kdim = (size_t)C * C;

@@ -50,7 +50,7 @@ struct PlaneInfo
     int sshift;  // log2 of the streams per workgroup, kGridThreads/64/wpp
     // First tap of this plane in the expanded tap table: K*K sub-cell
     // tables of W*W taps each, tap (j, k) at [(iv*K + iu)*W*W + j*W + k]
-    // (tables are limited to 8 GiB, so the offset fits 32 bits).
+    // (tables are limited to 4 GiB, so byte offsets fit 32 bits).
     unsigned eoff;
     // Rank of S among the distinct supports: planes of one rank share the
     // footprint shape, and the scatter takes their points as one segment.
