@@ -48,6 +48,8 @@ EXPORTED_SYMBOLS = (
     "oskar_grid_weights_write_f", "oskar_grid_weights_read_f",
     "oskar_grid_weights_write_d", "oskar_grid_weights_read_d",
     "gw_grid_weights_write_device", "gw_grid_weights_read_device",
+    "gw_plan_create_device_cube", "gw_wkernel_generate",
+    "gw_server_add_wkernel_set",
 )
 
 
@@ -216,6 +218,16 @@ def lib():
         ct.c_int, _sz, _vp, _vp, _vp, _vp, ct.c_double, ct.c_int, _szp, _vp,
         _vp, _ip]
     L.gw_grid_weights_read_device.restype = None
+    L.gw_plan_create_device_cube.argtypes = [ct.c_int, _sz, _i32p, ct.c_int,
+                                             ct.c_int, _vp, ct.c_double,
+                                             ct.c_double, ct.c_int, _ip]
+    L.gw_plan_create_device_cube.restype = _vp
+    L.gw_wkernel_generate.argtypes = [_sz, _i32p, ct.c_int, ct.c_int,
+                                      ct.c_double, _vp, _vp, _ip]
+    L.gw_wkernel_generate.restype = None
+    L.gw_server_add_wkernel_set.argtypes = [_vp, _sz, _i32p, ct.c_int,
+                                            ct.c_int, ct.c_double, _ip]
+    L.gw_server_add_wkernel_set.restype = ct.c_int
     _lib = L
     return L
 
@@ -398,6 +410,33 @@ def grid_wproj_f_diagnostics(num_w_planes, support, oversample,
     return int(sk.value), float(nrm.value)
 
 
+def wkernel_generate_device(support, oversample, conv_size_half, w_max,
+                            d_out=None):
+    """gw_wkernel_generate into a DeviceArray (allocated if not given);
+    returns it."""
+    sup = _c(support, np.int32)
+    n = 2 * sup.size * conv_size_half * conv_size_half
+    if d_out is None:
+        d_out = DeviceArray(4 * n)
+    st = ct.c_int(0)
+    lib().gw_wkernel_generate(sup.size, sup, oversample, conv_size_half,
+                              float(w_max), d_out.ptr, None, ct.byref(st))
+    _check(st, "gw_wkernel_generate")
+    return d_out
+
+
+def wkernel_generate(support, oversample, conv_size_half, w_max):
+    """The device-generated W-kernel cube as a host float32 array
+    [2 * P * C * C] (reference layout)."""
+    sup = _c(support, np.int32)
+    d = wkernel_generate_device(sup, oversample, conv_size_half, w_max)
+    try:
+        return d.download(np.empty(2 * sup.size * conv_size_half ** 2,
+                                   np.float32))
+    finally:
+        d.close()
+
+
 def grid_weights_write(uu, vv, weight, cell_size_rad, grid_size, grid,
                        double=False):
     """oskar_grid_weights_write_f / _d on host arrays: adds into ``grid``
@@ -568,6 +607,23 @@ class Plan:
         _check(st, "gw_plan_create")
 
     @classmethod
+    def from_device_cube(cls, precision, num_w_planes, support, oversample,
+                         conv_size_half, d_conv_func, cell_size_rad, w_scale,
+                         grid_size):
+        """gw_plan_create_device_cube: tables built from a cube already in
+        device memory (a DeviceArray)."""
+        self = cls.__new__(cls)
+        st = ct.c_int(0)
+        self.ptr = lib().gw_plan_create_device_cube(
+            precision, num_w_planes, _c(support, np.int32), oversample,
+            conv_size_half, _ptr(d_conv_func), cell_size_rad, w_scale,
+            grid_size, ct.byref(st))
+        _check(st, "gw_plan_create_device_cube")
+        self.precision = precision
+        self.grid_size = grid_size
+        return self
+
+    @classmethod
     def for_workload(cls, wl, precision=GW_SINGLE):
         return cls(precision, wl.num_w_planes, wl.support, wl.oversample,
                    wl.conv_size_half, wl.conv_func, wl.cell_size_rad,
@@ -715,6 +771,16 @@ class Server:
             self.ptr, num_w_planes, _c(support, np.int32), oversample,
             conv_size_half, _c(conv_func, np.float32), ct.byref(st))
         _check(st, "gw_server_add_kernel_set")
+        return k
+
+    def add_wkernel_set(self, support, oversample, conv_size_half, w_max):
+        """A kernel set generated on the device by its parameters."""
+        st = ct.c_int(0)
+        sup = _c(support, np.int32)
+        k = lib().gw_server_add_wkernel_set(self.ptr, sup.size, sup,
+                                            oversample, conv_size_half,
+                                            float(w_max), ct.byref(st))
+        _check(st, "gw_server_add_wkernel_set")
         return k
 
     def create_grid(self, grid_size):

@@ -20,7 +20,8 @@ LIB = os.path.join(HERE, "libfpga_gridding_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GW_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["gridder.hip", "weights.hip", "capi.cpp", "server.cpp"]
+SOURCES = ["gridder.hip", "weights.hip", "wkernel.hip", "capi.cpp",
+           "server.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -587,6 +587,40 @@ gw_plan* gw_plan_create(int precision, size_t num_w_planes,
     return out;
 }
 
+GW_EXPORT
+gw_plan* gw_plan_create_device_cube(int precision, size_t num_w_planes,
+        const int* support, int oversample, int conv_size_half,
+        const float* d_conv_func, double cell_size_rad, double w_scale,
+        int grid_size, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return nullptr;
+    gw_plan* out = nullptr;
+    guarded(status, [&] {
+        std::unique_ptr<gw::Plan> p(new gw::Plan(precision, num_w_planes,
+                support, oversample, conv_size_half, d_conv_func,
+                cell_size_rad, w_scale, grid_size, true));
+        out = new gw_plan{p.release()};
+    });
+    return out;
+}
+
+GW_EXPORT
+void gw_wkernel_generate(size_t num_w_planes, const int* support,
+        int oversample, int conv_size_half, double w_max, float* d_conv_func,
+        void* hip_stream, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        gw::generate_wkernels(num_w_planes, support, oversample,
+                conv_size_half, w_max, d_conv_func,
+                static_cast<hipStream_t>(hip_stream));
+    });
+}
+
 GW_EXPORT void gw_plan_free(gw_plan* plan)
 {
     if (!plan) return;
@@ -997,6 +1031,22 @@ GW_EXPORT int gw_server_add_kernel_set(gw_server* server, size_t num_w_planes,
         if (!server) gw::fail(gw::kErrInvalidArgument, "server is NULL");
         id = server->impl->add_kernel_set(num_w_planes, support, oversample,
                                           conv_size_half, conv_func);
+    });
+    return id;
+}
+
+GW_EXPORT int gw_server_add_wkernel_set(gw_server* server,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, double w_max, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return -1;
+    int id = -1;
+    guarded(status, [&] {
+        if (!server) gw::fail(gw::kErrInvalidArgument, "server is NULL");
+        id = server->impl->add_wkernel_set(num_w_planes, support, oversample,
+                                           conv_size_half, w_max);
     });
     return id;
 }

@@ -1358,6 +1358,57 @@ void build_tables(const T* conv, size_t P, const int* support, int ov, int C,
 
 }  // namespace
 
+// The expanded tap table and the tap-sum table built on the device from a
+// float cube already in device memory: the same gathers as build_tables,
+// and the same sums in the same order (rows j outer, columns k inner,
+// double accumulation of float taps), so both tables are bit-identical to
+// the host build of the same cube.
+__global__ void __launch_bounds__(256) k_expand_taps(
+        const float* __restrict__ conv, int C, int ov, int h, int K,
+        const int* __restrict__ support, const PlaneInfo* __restrict__ pinfo,
+        float* __restrict__ etab)
+{
+    const int p = blockIdx.y;
+    const int S = support[p];
+    const int W = 2 * S + 1;
+    const unsigned n = (unsigned)(K * K * W * W);
+    const size_t base = (size_t)p * C * C;
+    const size_t e0 = pinfo[p].eoff;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += gridDim.x * blockDim.x)
+    {
+        const int combo = (int)(t / (unsigned)(W * W));
+        const int jk = (int)(t - (unsigned)combo * W * W);
+        const int iv = combo / K, iu = combo - iv * K;
+        const int j = jk / W - S, k = jk - (jk / W) * W - S;
+        const size_t src = 2 * (base + (size_t)abs(iv - h + j * ov) * C +
+                (size_t)abs(iu - h + k * ov));
+        etab[2 * (e0 + t)] = conv[src];
+        etab[2 * (e0 + t) + 1] = conv[src + 1];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_tap_sums(
+        const float* __restrict__ conv, int P, int C, int ov, int h, int K,
+        const int* __restrict__ support, double* __restrict__ normtab)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P * K * K) return;
+    const int p = i / (K * K);
+    const int combo = i - p * K * K;
+    const int iv = combo / K, iu = combo - iv * K;
+    const int S = support[p];
+    const size_t base = (size_t)p * C * C;
+    double sum = 0.0;
+    for (int j = -S; j <= S; ++j)
+    {
+        const size_t t1 = base + (size_t)abs(iv - h + j * ov) * C;
+        for (int k = -S; k <= S; ++k)
+            sum += conv[2 * (t1 + (size_t)abs(iu - h + k * ov))];
+    }
+    normtab[i] = sum;
+}
+
 void validate_args(size_t num_w_planes, const int* support, int oversample,
                    int conv_size_half, const void* conv_func, int grid_size)
 {
@@ -1392,7 +1443,8 @@ void validate_args(size_t num_w_planes, const int* support, int oversample,
 
 Plan::Plan(int precision, size_t num_w_planes, const int* support,
            int oversample, int conv_size_half, const void* conv_func,
-           double cell_size_rad, double w_scale, int grid_size)
+           double cell_size_rad, double w_scale, int grid_size,
+           bool cube_on_device)
 {
     if (precision < kPrecSingle || precision > kPrecSingleIndex64)
         fail(kErrInvalidArgument, "precision must be GW_SINGLE, GW_DOUBLE, "
@@ -1460,6 +1512,11 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
         check_hip(hipEventCreate(&e), kErrComputeDevices, "hipEventCreate");
 
     // Constant tables.
+    if (cube_on_device)
+    {
+        build_tables_device(static_cast<const float*>(conv_func));
+        return;
+    }
     std::vector<double> normtab;
     std::vector<float> etab_f;
     std::vector<double> etab_d;
@@ -1542,6 +1599,56 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     check_hip(hipMemcpy(d_normtab_, normtab.data(),
             normtab.size() * sizeof(double), hipMemcpyHostToDevice),
             kErrMemCopy, "upload norm table");
+}
+
+void Plan::build_tables_device(const float* d_conv)
+{
+    if (prec_ != kPrecSingle)
+        fail(kErrInvalidArgument, "a kernel cube in device memory needs a "
+             "GW_SINGLE or GW_SINGLE_INDEX64 plan");
+    size_t total = 0;
+    size_t most = 0;
+    for (size_t p = 0; p < P_; ++p)
+    {
+        const size_t W = 2 * (size_t)support_[p] + 1;
+        pinfo_[p].eoff = (unsigned)total;
+        total += (size_t)K_ * K_ * W * W;
+        most = std::max(most, (size_t)K_ * K_ * W * W);
+    }
+    if (total * 2 * sizeof(float) >= (size_t)4 << 30)
+        fail(kErrInvalidArgument,
+             "support and oversample need 4 GiB or more of tap tables");
+    etab_taps_ = total;
+    const size_t etab_bytes = total * 2 * sizeof(float);
+    const size_t ntab = P_ * (size_t)K_ * K_;
+    d_etab_ = dmalloc<char>(etab_bytes);
+    table_bytes_ = etab_bytes + P_ * (sizeof(int) + sizeof(PlaneInfo)) +
+            ntab * sizeof(double);
+    d_support_ = dmalloc<int>(P_);
+    d_srank_ = dmalloc<int>(P_);
+    d_pinfo_ = dmalloc<PlaneInfo>(P_);
+    d_normtab_ = dmalloc<double>(ntab);
+    d_combo_ = dmalloc<unsigned>(ntab);
+    d_summary_ = dmalloc<Summary>(1);
+    check_hip(hipHostMalloc((void**)&h_summary_, sizeof(Summary),
+            hipHostMallocDefault), kErrMemAlloc, "hipHostMalloc");
+    std::memset(h_summary_, 0, sizeof(Summary));
+    check_hip(hipMemcpy(d_support_, support_.data(), P_ * sizeof(int),
+            hipMemcpyHostToDevice), kErrMemCopy, "upload support");
+    check_hip(hipMemcpy(d_srank_, srank_.data(), P_ * sizeof(int),
+            hipMemcpyHostToDevice), kErrMemCopy, "upload support ranks");
+    check_hip(hipMemcpy(d_pinfo_, pinfo_.data(), P_ * sizeof(PlaneInfo),
+            hipMemcpyHostToDevice), kErrMemCopy, "upload plane table");
+    const unsigned bx = (unsigned)std::min<size_t>((most + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_expand_taps, dim3(bx, (unsigned)P_), dim3(256), 0,
+            stream_, d_conv, C_, ov_, h_, K_, (const int*)d_support_,
+            (const PlaneInfo*)d_pinfo_, reinterpret_cast<float*>(d_etab_));
+    hipLaunchKernelGGL(k_tap_sums, dim3((unsigned)((ntab + 255) / 256)),
+            dim3(256), 0, stream_, d_conv, (int)P_, C_, ov_, h_, K_,
+            (const int*)d_support_, d_normtab_);
+    check_hip(hipGetLastError(), kErrKernelLaunch, "table build launch");
+    check_hip(hipStreamSynchronize(stream_), kErrKernelLaunch,
+            "stream synchronize");
 }
 
 int Plan::max_tile_bits() const

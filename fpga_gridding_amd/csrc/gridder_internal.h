@@ -217,7 +217,8 @@ class Plan
 public:
     Plan(int precision, size_t num_w_planes, const int* support,
          int oversample, int conv_size_half, const void* conv_func,
-         double cell_size_rad, double w_scale, int grid_size);
+         double cell_size_rad, double w_scale, int grid_size,
+         bool cube_on_device = false);
     ~Plan();
 
     // Stage 1 (enqueue): index math, skip count, histograms, sort keys.
@@ -278,6 +279,7 @@ public:
 
 private:
     void ensure_capacity(size_t n);
+    void build_tables_device(const float* d_conv);
     void free_workspace();
     template <typename T, typename TU = T, typename TV = T,
               typename Rec = typename Types<T>::Rec>
@@ -508,6 +510,13 @@ private:
     unsigned long long* d_small_ = nullptr;   // skipped; scan length, total
     unsigned long long* h_skipped_ = nullptr;
 };
+
+// W-kernel cube generated on the device (float, the reference cube's
+// layout) from the plane supports, oversample, cube half size and w_max;
+// enqueued on `stream`, waits.
+void generate_wkernels(size_t num_w_planes, const int* support,
+                       int oversample, int conv_size_half, double w_max,
+                       float* d_out, hipStream_t stream);
 
 // Zero-fill with a kernel (capture-safe, unlike hipMemsetAsync).
 void fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);

@@ -66,6 +66,8 @@ Server::~Server()
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     for (Slot& s : slot_)
         if (s.stage) (void)hipFree(s.stage);
+    plans_.clear();
+    for (KernelSet& k : ksets_) dfree(k.d_cube);
 }
 
 int Server::add_kernel_set(size_t num_w_planes, const int* support,
@@ -81,6 +83,43 @@ int Server::add_kernel_set(size_t num_w_planes, const int* support,
     k.C = conv_size_half;
     k.cube.assign(conv_func, conv_func + 2 * num_w_planes *
                   (size_t)conv_size_half * conv_size_half);
+    std::lock_guard<std::mutex> g(mu_);
+    ksets_.push_back(std::move(k));
+    return (int)ksets_.size() - 1;
+}
+
+int Server::add_wkernel_set(size_t num_w_planes, const int* support,
+                            int oversample, int conv_size_half, double w_max)
+{
+    validate_args(num_w_planes, support, oversample, conv_size_half,
+                  support, 1);
+    const size_t bytes = 2 * num_w_planes * (size_t)conv_size_half *
+            conv_size_half * sizeof(float);
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        if (used_ + bytes > budget_)
+            fail(kErrMemAlloc, "kernel cube exceeds the HBM budget");
+        used_ += bytes;
+    }
+    KernelSet k;
+    k.P = num_w_planes;
+    k.support.assign(support, support + num_w_planes);
+    k.ov = oversample;
+    k.C = conv_size_half;
+    try
+    {
+        (void)hipSetDevice(device_);
+        k.d_cube = dmalloc<float>(bytes / sizeof(float));
+        generate_wkernels(num_w_planes, support, oversample, conv_size_half,
+                          w_max, k.d_cube, nullptr);
+    }
+    catch (...)
+    {
+        dfree(k.d_cube);
+        std::lock_guard<std::mutex> g(mu_);
+        used_ -= bytes;
+        throw;
+    }
     std::lock_guard<std::mutex> g(mu_);
     ksets_.push_back(std::move(k));
     return (int)ksets_.size() - 1;
@@ -219,9 +258,12 @@ Plan* Server::plan_for(const Request& r, size_t* new_bytes)
         return it->second.plan.get();
     }
     const KernelSet& k = ksets_[r.kernel_set];
-    std::unique_ptr<Plan> p(new Plan(precision_, k.P, k.support.data(),
-            k.ov, k.C, k.cube.data(), r.cell_size_rad, r.w_scale,
-            r.grid_size));
+    std::unique_ptr<Plan> p(k.d_cube ?
+            new Plan(precision_, k.P, k.support.data(), k.ov, k.C, k.d_cube,
+                     r.cell_size_rad, r.w_scale, r.grid_size, true) :
+            new Plan(precision_, k.P, k.support.data(), k.ov, k.C,
+                     k.cube.data(), r.cell_size_rad, r.w_scale,
+                     r.grid_size));
     *new_bytes = p->table_bytes();
     Plan* raw = p.get();
     plans_[key] = CachedPlan{std::move(p), ++use_clock_, 0};

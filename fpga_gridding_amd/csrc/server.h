@@ -61,6 +61,10 @@ public:
     int add_kernel_set(size_t num_w_planes, const int* support,
                        int oversample, int conv_size_half,
                        const float* conv_func);
+    // A kernel set generated on the device (generate_wkernels); its cube
+    // stays in device memory and plans are built from it there.
+    int add_wkernel_set(size_t num_w_planes, const int* support,
+                        int oversample, int conv_size_half, double w_max);
     int create_grid(int grid_size);
     void read_grid(int grid_id, float* host_out);
     void free_grid(int grid_id);
@@ -76,7 +80,8 @@ private:
         size_t P = 0;
         std::vector<int> support;
         int ov = 0, C = 0;
-        std::vector<float> cube;
+        std::vector<float> cube;     // host cube, or
+        float* d_cube = nullptr;     // a cube in device memory
     };
     typedef std::tuple<int, int, double, double> PlanKey;
     struct CachedPlan

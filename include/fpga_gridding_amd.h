@@ -114,6 +114,32 @@ gw_plan* gw_plan_create(int precision, size_t num_w_planes,
         const void* conv_func, double cell_size_rad, double w_scale,
         int grid_size, int* status);
 
+/* As gw_plan_create for GW_SINGLE / GW_SINGLE_INDEX64, with the float
+ * kernel cube already in DEVICE memory (for example from
+ * gw_wkernel_generate): the tap tables are built on the device, no cube
+ * crosses PCIe.  The tables equal those gw_plan_create builds from the same
+ * values. */
+GW_EXPORT
+gw_plan* gw_plan_create_device_cube(int precision, size_t num_w_planes,
+        const int* support, int oversample, int conv_size_half,
+        const float* d_conv_func, double cell_size_rad, double w_scale,
+        int grid_size, int* status);
+
+/* Generate a W-kernel cube in DEVICE memory (d_conv_func: 2*num_w_planes*
+ * conv_size_half^2 floats, the reference layout): per plane p a Gaussian
+ * taper exp(-r^2 / (2 sigma^2)), sigma = max(S_p, 1) / 2.5 pixels, times
+ * the chirp exp(i pi w_p s r^2), w_p = (p / (num_w_planes - 1))^2 w_max,
+ * s = 2 / (w_max max(S)^2), on the quadrant samples r^2 = (ix^2 + iy^2) /
+ * oversample^2, normalised to unit complex sum over the footprint taps at
+ * offset zero, computed in FP64 and rounded to float.  No reference
+ * counterpart (the reference reads precomputed FITS kernels,
+ * read_kernel.c:36-118).  Enqueued on hip_stream (NULL: the null stream);
+ * returns after completion. */
+GW_EXPORT
+void gw_wkernel_generate(size_t num_w_planes, const int* support,
+        int oversample, int conv_size_half, double w_max, float* d_conv_func,
+        void* hip_stream, int* status);
+
 GW_EXPORT void gw_plan_free(gw_plan* plan);
 
 /* Use the given hipStream_t for the plan's work (NULL: the plan's own
@@ -278,6 +304,12 @@ GW_EXPORT void gw_server_free(gw_server* server);
 GW_EXPORT int gw_server_add_kernel_set(gw_server* server, size_t num_w_planes,
         const int* support, int oversample, int conv_size_half,
         const float* conv_func, int* status);
+/* Registers a kernel set generated on the device by its parameters (as
+ * gw_wkernel_generate); the cube stays in device memory (counted against
+ * the budget) and plans are built from it there.  Returns its id. */
+GW_EXPORT int gw_server_add_wkernel_set(gw_server* server,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, double w_max, int* status);
 /* A zeroed resident grid (grid_size^2 complex64) that requests accumulate
  * into; returns its id.  Reading waits until no queued request uses it. */
 GW_EXPORT int gw_server_create_grid(gw_server* server, int grid_size,
