@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GW_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["gridder.hip", "weights.hip", "wkernel.hip", "capi.cpp",
-           "server.cpp"]
+           "server.cpp", "io.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -28,7 +28,8 @@ def _declared_symbols():
         text = "\n".join(l for l in text.splitlines()
                          if not l.lstrip().startswith("#"))
         # GW_EXPORT <ret> <name>(   possibly across lines
-        for m in re.finditer(r"GW_EXPORT\s+[\w\s\*]*?\b(\w+)\s*\(", text):
+        for m in re.finditer(r"GW_(?:IO_)?EXPORT\s+[\w\s\*]*?\b(\w+)\s*\(",
+                             text):
             names.add(m.group(1))
     return names
 
@@ -43,7 +44,8 @@ def test_library_exports_every_declared_symbol():
     lib = ct.CDLL(fg.LIB_PATH)
     for name in sorted(declared):
         assert hasattr(lib, name), name
-    assert declared == set(fg.EXPORTED_SYMBOLS)
+    from fpga_gridding_amd import io as gio
+    assert declared == set(fg.EXPORTED_SYMBOLS) | set(gio.IO_SYMBOLS)
 
 
 def test_error_strings_and_codes():

@@ -1,0 +1,254 @@
+"""Host I/O of the library (no GPU): OSKAR binary visibility files, FITS
+kernel cubes, FITS grid cubes.
+
+* OSKAR binary: files written by the reference's own binary-file library
+  (tests/golden/make_vis_golden.py, a gcc build of utils/binary/src) are
+  read back bit-exactly -- header ints and doubles, block arrays,
+  coordinates through gw_vis_read_coords (read_coords, read_vis.c:30-46),
+  an extended tag; a corrupted payload fails its CRC-32C; missing tags and
+  short buffers give the reference's status codes.
+* scale_coords (read_vis.c:49-73): bit-exact against the formula (FP64
+  scale, rounded to float for float coordinates).
+* FITS: kernel cubes built here byte by byte from the FITS standard (cards,
+  big-endian data, BINTABLE extension) are read as read_kernel reads them
+  (read_kernel.c:35-118); cubes written by gw_write_fits_cube
+  (write_fits_cube.c) are parsed back with an independent reader, plane by
+  plane.  No reference FITS build exists here (it needs cfitsio), so these
+  are pinned by the standard, not by reference files.
+"""
+import os
+import shutil
+import struct
+
+import numpy as np
+import pytest
+
+from fpga_gridding_amd import io as gio
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def _bits(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.dtype == b.dtype and a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("which,prec", [("f", gio.OSKAR_SINGLE),
+                                        ("d", gio.OSKAR_DOUBLE)])
+def test_reads_reference_written_vis_file(which, prec):
+    ref = np.load(os.path.join(GOLD, "ref_vis.npz"))
+    g = lambda k: ref[f"{which}_{k}"]
+    with gio.BinaryFile(os.path.join(GOLD, f"ref_vis_{which}.vis")) as f:
+        H, B = gio.VIS_GROUP_HEADER, gio.VIS_GROUP_BLOCK
+        names = dict(num_tags_per_block=2, amp_type=5, coord_precision=6,
+                     max_times_per_block=7, num_times_total=8,
+                     num_channels_total=10, num_stations=11)
+        for k, t in names.items():
+            assert f.read_int(H, t) == int(g("hdr_" + k)), k
+        assert f.read_double(H, 23) == float(g("hdr_freq_start_hz"))
+        assert f.read_int(H, 6) == prec
+        for b in range(2):
+            dims = f.read(gio.OSKAR_INT, B, 1, b, np.int32)
+            assert _bits(dims, g(f"b{b}_dims"))
+            vis = f.read(gio.OSKAR_SINGLE | gio.OSKAR_COMPLEX, B, 3, b,
+                         np.float32)
+            assert _bits(vis, g(f"b{b}_vis"))
+            n = int(dims[2] * dims[4])
+            uu, vv, ww = f.read_coords(b, prec, n)
+            assert _bits(uu, g(f"b{b}_uu")) and _bits(vv, g(f"b{b}_vv"))
+            assert _bits(ww, g(f"b{b}_ww"))
+        note = g("ext_note")
+        assert _bits(f.read_ext(gio.OSKAR_CHAR, "fixture", "note", 7,
+                                note.size), note)
+        assert _bits(f.read(gio.OSKAR_DOUBLE, 1, 99, 3, np.float64),
+                     g("extra_double"))
+        # header ints (7) + double + 2 blocks x 5 tags + ext + extra
+        assert f.num_tags() == 7 + 1 + 10 + 1 + 1
+
+
+def test_vis_errors_and_crc(tmp_path):
+    src = os.path.join(GOLD, "ref_vis_f.vis")
+    with gio.BinaryFile(src) as f:
+        with pytest.raises(gio.IOError_) as e:
+            f.read_int(12, 77)
+        assert e.value.status == -115            # tag not found
+        with pytest.raises(gio.IOError_) as e:
+            f.read(gio.OSKAR_INT, 12, 1, 0, np.int32, count=2)
+        assert e.value.status == -114            # buffer too small
+        with pytest.raises(gio.IOError_) as e:
+            f.read(gio.OSKAR_DOUBLE, 12, 4, 0, np.float64)  # wrong type
+        assert e.value.status == -115
+    # Flip one byte inside the first block's uu payload: its CRC fails.
+    bad = str(tmp_path / "bad.vis")
+    shutil.copy(src, bad)
+    raw = bytearray(open(bad, "rb").read())
+    ref = np.load(os.path.join(GOLD, "ref_vis.npz"))
+    at = bytes(raw).find(ref["f_b0_uu"].tobytes())
+    assert at > 0
+    raw[at + 5] ^= 0x40
+    open(bad, "wb").write(bytes(raw))
+    with gio.BinaryFile(bad) as f:
+        with pytest.raises(gio.IOError_) as e:
+            f.read_coords(0, gio.OSKAR_SINGLE, ref["f_b0_uu"].size)
+        assert e.value.status == -118
+        f.read_coords(1, gio.OSKAR_SINGLE, ref["f_b1_uu"].size)  # intact
+    with pytest.raises(gio.IOError_) as e:
+        gio.BinaryFile(str(tmp_path / "missing.vis"))
+    assert e.value.status == -100
+    junk = str(tmp_path / "junk.vis")
+    open(junk, "wb").write(b"NOTOSKAR" + bytes(100))
+    with pytest.raises(gio.IOError_) as e:
+        gio.BinaryFile(junk)
+    assert e.value.status == -106
+
+
+def test_scale_coords_matches_formula():
+    rng = np.random.default_rng(3)
+    freq = 150.3e6
+    scale = freq / 299792458.0
+    for dt in (np.float32, np.float64):
+        u0 = (rng.normal(size=1000) * 3000).astype(dt)
+        u, v, w = u0.copy(), (u0 * 2).astype(dt), (u0 * -0.5).astype(dt)
+        gio.scale_coords(freq, u, v, w)
+        s = dt(scale)
+        assert _bits(u, u0 * s) and _bits(v, (u0 * 2).astype(dt) * s)
+        assert _bits(w, (u0 * -0.5).astype(dt) * s)
+
+
+# ---------------------------------------------------------------------------
+# FITS, written and parsed here from the standard
+# ---------------------------------------------------------------------------
+
+def _cards(pairs):
+    out = b""
+    for k, v in pairs:
+        if k == "END":
+            out += b"END".ljust(80)
+            continue
+        out += (k.ljust(8) + "= " + str(v).rjust(20)).ljust(80).encode()
+    return out.ljust(((len(out) + 2879) // 2880) * 2880, b" ")
+
+
+def _pad(b):
+    return b.ljust(((len(b) + 2879) // 2880) * 2880, b"\0")
+
+
+def _kernel_fits(path, cube, support, bitpix=-32, extra=(), tform="1J"):
+    P, C, _ = cube.shape
+    hdr = [("SIMPLE", "T"), ("BITPIX", bitpix), ("NAXIS", 3),
+           ("NAXIS1", C), ("NAXIS2", C), ("NAXIS3", P), ("EXTEND", "T"),
+           ("OVERSAMP", 4), ("GRIDSIZE", 2048),
+           ("CELLSIZE", "1.7044230976507E-05"), ("W_SCALE", "0.25D+01"),
+           *extra, ("END", None)]
+    fmt = {-32: ">f4", -64: ">f8", 16: ">i2", 32: ">i4"}[bitpix]
+    data = _pad(np.ascontiguousarray(cube).astype(fmt).tobytes())
+    code, width = {"1J": (">i4", 4), "1E": (">f4", 4), "1I": (">i2", 2)}[tform]
+    rows = np.asarray(support).astype(code).tobytes()
+    tab = [("XTENSION", "'BINTABLE'"), ("BITPIX", 8), ("NAXIS", 2),
+           ("NAXIS1", width), ("NAXIS2", len(support)), ("PCOUNT", 0),
+           ("GCOUNT", 1), ("TFIELDS", 1), ("TTYPE1", "'SUPPORT '"),
+           ("TFORM1", f"'{tform}      '"), ("EXTNAME", "'W_KERNELS'"),
+           ("END", None)]
+    open(path, "wb").write(_cards(hdr) + data + _cards(tab) + _pad(rows))
+
+
+def test_read_kernel_float_cube_and_support(tmp_path):
+    rng = np.random.default_rng(4)
+    cube = rng.normal(size=(3, 9, 9)).astype(np.float32)
+    sup = np.array([1, 2, 2], np.int32)
+    path = str(tmp_path / "k.fits")
+    _kernel_fits(path, cube, sup)
+    got, hdr, s = gio.read_kernel(path)
+    assert _bits(got, cube) and np.array_equal(s, sup)
+    assert hdr == dict(conv_size_half=9, num_w_planes=3, oversample=4,
+                       grid_size=2048, cellsize_rad=1.7044230976507e-05,
+                       w_scale=2.5)
+    got_d, _, _ = gio.read_kernel(path, gio.OSKAR_DOUBLE, support=False)
+    assert _bits(got_d, cube.astype(np.float64))
+
+
+def test_read_kernel_conversions(tmp_path):
+    rng = np.random.default_rng(5)
+    cube = rng.normal(size=(2, 5, 5))
+    path = str(tmp_path / "k64.fits")
+    _kernel_fits(path, cube, [1, 2], bitpix=-64, tform="1E")
+    got, _, s = gio.read_kernel(path)
+    assert _bits(got, cube.astype(np.float32)) and list(s) == [1, 2]
+    ints = rng.integers(-300, 300, size=(2, 5, 5))
+    path = str(tmp_path / "k16.fits")
+    _kernel_fits(path, ints, [3, 4], bitpix=16, tform="1I",
+                 extra=(("BSCALE", "0.5"), ("BZERO", "1.0")))
+    got, _, s = gio.read_kernel(path, gio.OSKAR_DOUBLE)
+    assert _bits(got, ints * 0.5 + 1.0) and list(s) == [3, 4]
+
+
+def test_read_kernel_errors(tmp_path):
+    cube = np.zeros((2, 5, 5), np.float32)
+    path = str(tmp_path / "nosup.fits")
+    P, C = 2, 5
+    hdr = [("SIMPLE", "T"), ("BITPIX", -32), ("NAXIS", 3), ("NAXIS1", C),
+           ("NAXIS2", C), ("NAXIS3", P), ("OVERSAMP", 4), ("GRIDSIZE", 64),
+           ("CELLSIZE", "1E-5"), ("W_SCALE", "1.0"), ("END", None)]
+    open(path, "wb").write(_cards(hdr) + _pad(cube.astype(">f4").tobytes()))
+    with pytest.raises(gio.IOError_):
+        gio.read_kernel(path)                        # no W_KERNELS table
+    gio.read_kernel(path, support=False)             # fine without it
+    hdr2 = [h for h in hdr if h[0] != "W_SCALE"]
+    open(path, "wb").write(_cards(hdr2) + _pad(cube.astype(">f4").tobytes()))
+    with pytest.raises(gio.IOError_):
+        gio.read_kernel(path, support=False)         # missing keyword
+    with pytest.raises(gio.IOError_):
+        gio.read_kernel(str(tmp_path / "none.fits"))
+
+
+def _parse_image(path):
+    """Independent reader: primary header cards and the big-endian image."""
+    raw = open(path, "rb").read()
+    cards, at = {}, 0
+    while True:
+        c = raw[at:at + 80].decode()
+        at += 80
+        if c.startswith("END"):
+            break
+        if c[8:10] == "= ":
+            cards[c[:8].strip()] = c[10:].split("/")[0].strip()
+    data_at = ((at + 2879) // 2880) * 2880
+    bp = int(cards["BITPIX"])
+    shape = tuple(int(cards[f"NAXIS{k}"]) for k in (3, 2, 1))
+    n = int(np.prod(shape))
+    a = np.frombuffer(raw, ">f4" if bp == -32 else ">f8", n, data_at)
+    assert len(raw) % 2880 == 0
+    return cards, a.reshape(shape)
+
+
+def test_write_fits_cube_complex_and_planes(tmp_path):
+    rng = np.random.default_rng(6)
+    W, H = 7, 5
+    grid = rng.normal(size=2 * W * H).astype(np.float32)
+    root = str(tmp_path / "vis_grid")
+    gio.write_fits_cube(grid, root, W, H, 1, 0, complex_=True)
+    c, re = _parse_image(root + "_REAL.fits")
+    _, im = _parse_image(root + "_IMAG.fits")
+    assert c["SIMPLE"] == "T" and c["BITPIX"] == "-32" and c["NAXIS"] == "3"
+    assert _bits(re.astype(np.float32).ravel(), grid[0::2])
+    assert _bits(im.astype(np.float32).ravel(), grid[1::2])
+    # A 3-plane double cube, then plane 1 rewritten in place.
+    cube = rng.normal(size=(3, H, W))
+    path = str(tmp_path / "cube.fits")
+    gio.write_fits_cube(cube, path, W, H, 3, -1)
+    _, a = _parse_image(path)
+    assert _bits(a.astype(np.float64), cube)
+    plane = rng.normal(size=(H, W))
+    gio.write_fits_cube(plane, path, W, H, 3, 1)
+    _, a = _parse_image(path)
+    assert _bits(a[1].astype(np.float64), plane)
+    assert _bits(a[0].astype(np.float64), cube[0])
+    assert _bits(a[2].astype(np.float64), cube[2])
+    # Other dimensions replace the file.
+    gio.write_fits_cube(plane.astype(np.float32), path, W, H, 1, 0)
+    c, a = _parse_image(path)
+    assert c["NAXIS3"] == "1" and c["BITPIX"] == "-32"
+    # ".fits" is appended unless present.
+    gio.write_fits_cube(plane, str(tmp_path / "img"), W, H, 1, 0)
+    assert os.path.exists(str(tmp_path / "img.fits"))
