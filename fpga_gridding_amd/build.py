@@ -43,7 +43,7 @@ FLAGS = [
 
 def build(verbose: bool = False) -> str:
     """Compile every source into the shared library; returns its path."""
-    objs = []
+    objs, cmds = [], []
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
     for src in SOURCES:
@@ -54,8 +54,15 @@ def build(verbose: bool = False) -> str:
             cmd[1:1] = ["-x", "hip"]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    # The sources compile independently: run them side by side.
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "4"))))
+    with ThreadPoolExecutor(jobs) as ex:
+        for r in ex.map(lambda c: subprocess.run(c), cmds):
+            if r.returncode != 0:
+                raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread",
            "-o", tmp, *objs]
