@@ -642,12 +642,12 @@ __global__ void __launch_bounds__(256) k_write_items(
 // -------------------------------------------------------------------------
 //
 // A job is up to kJobPoints sorted points of one T x T tile of footprint
-// origins, ordered by plane, then position, then sub-cell table.  The
-// workgroup accumulates them into an LDS copy of the tile's
+// origins, ordered by support, then position, then plane, then sub-cell
+// table.  The workgroup accumulates them into an LDS copy of the tile's
 // (T + Wmax - 1)^2 grid region (planar real / imaginary arrays), across
 // planes, and adds the region to the grid in HBM once at the end of the job.
 //
-// Inside a plane segment the waves form streams of wpp waves; each stream
+// Inside a support segment the waves form streams of wpp waves; each stream
 // takes a contiguous chunk of the segment's points (at least min_share
 // points; chunk q goes to stream (q + first point of the segment) mod
 // streams, so short segments spread over all streams).  Lane slot s of wave
@@ -657,7 +657,8 @@ __global__ void __launch_bounds__(256) k_write_items(
 // (= conv_func[p][|off_v + (j-S)*ov|][|off_u + (k-S)*ov|]).  The tap stays in
 // registers while consecutive points share the plane and table; the products
 // with the visibilities are summed in registers while consecutive points
-// share the position (whatever their tables); when the position changes the
+// share the position (whatever their planes and tables: all planes of a
+// segment have the same footprint shape); when the position changes the
 // slot adds its sum to the region with an LDS atomic.  For planes whose
 // region would not fit in LDS the sums go straight to the grid in HBM.
 
