@@ -98,6 +98,7 @@ struct gw_binary
 {
     FILE* f = nullptr;
     int version = 0;
+    int search_start = 0;     // first chunk a query looks at
     struct Chunk
     {
         bool extended;
@@ -552,6 +553,25 @@ GW_IO_EXPORT int gw_binary_num_tags(const gw_binary* h)
     return h ? (int)h->chunks.size() : 0;
 }
 
+GW_IO_EXPORT void gw_binary_set_query_search_start(gw_binary* h, int start,
+        int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    if (!h)
+    {
+        *status = kErrInvalidArgument;
+        return;
+    }
+    if (start < 0 || start >= (int)h->chunks.size())
+    {
+        *status = kBinTagOutOfRange;
+        return;
+    }
+    h->search_start = start;
+}
+
 GW_IO_EXPORT int gw_binary_query(const gw_binary* h, unsigned char data_type,
         unsigned char id_group, unsigned char id_tag, int user_index,
         size_t* payload_size, int* status)
@@ -564,7 +584,7 @@ GW_IO_EXPORT int gw_binary_query(const gw_binary* h, unsigned char data_type,
         *status = kErrInvalidArgument;
         return -1;
     }
-    for (int i = 0; i < (int)h->chunks.size(); ++i)
+    for (int i = h->search_start; i < (int)h->chunks.size(); ++i)
     {
         const gw_binary::Chunk& c = h->chunks[i];
         if (!c.extended && (c.data_type == data_type || !data_type) &&
@@ -609,7 +629,7 @@ GW_IO_EXPORT void gw_binary_read_ext(gw_binary* h, unsigned char data_type,
         *status = kBinTagTooLong;
         return;
     }
-    for (int i = 0; i < (int)h->chunks.size(); ++i)
+    for (int i = h->search_start; i < (int)h->chunks.size(); ++i)
     {
         const gw_binary::Chunk& c = h->chunks[i];
         if (c.extended && (c.data_type == data_type || !data_type) &&

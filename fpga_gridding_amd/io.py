@@ -14,6 +14,7 @@ VIS_GROUP_HEADER, VIS_GROUP_BLOCK = 11, 12
 
 IO_SYMBOLS = (
     "gw_binary_open", "gw_binary_free", "gw_binary_num_tags",
+    "gw_binary_set_query_search_start",
     "gw_binary_query", "gw_binary_read", "gw_binary_read_ext",
     "gw_binary_read_int", "gw_binary_read_double", "gw_vis_read_coords",
     "gw_vis_scale_coords", "gw_read_kernel", "gw_free",
@@ -42,6 +43,8 @@ def io_lib():
         L.gw_binary_free.restype = None
         L.gw_binary_num_tags.argtypes = [_vp]
         L.gw_binary_num_tags.restype = ct.c_int
+        L.gw_binary_set_query_search_start.argtypes = [_vp, ct.c_int, _ip]
+        L.gw_binary_set_query_search_start.restype = None
         L.gw_binary_query.argtypes = [_vp, ct.c_ubyte, ct.c_ubyte, ct.c_ubyte,
                                       ct.c_int, ct.POINTER(ct.c_size_t), _ip]
         L.gw_binary_query.restype = ct.c_int
@@ -93,6 +96,12 @@ class BinaryFile:
 
     def num_tags(self):
         return io_lib().gw_binary_num_tags(self.h)
+
+    def set_query_search_start(self, start):
+        st = ct.c_int(0)
+        io_lib().gw_binary_set_query_search_start(self.h, start,
+                                                  ct.byref(st))
+        _check(st, "gw_binary_set_query_search_start")
 
     def payload_size(self, data_type, group, tag, index=0):
         st = ct.c_int(0)

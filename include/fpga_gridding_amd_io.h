@@ -58,6 +58,11 @@ typedef struct gw_binary gw_binary;
 GW_IO_EXPORT gw_binary* gw_binary_open(const char* filename, int* status);
 GW_IO_EXPORT void gw_binary_free(gw_binary* handle);
 GW_IO_EXPORT int gw_binary_num_tags(const gw_binary* handle);
+/* Queries look at chunks from index start on (oskar_binary_set_query_
+ * search_start, used by main.cpp:232 and :422 to start each block's reads at its
+ * own tags). */
+GW_IO_EXPORT void gw_binary_set_query_search_start(gw_binary* handle,
+        int start, int* status);
 /* Index of the first standard tag with these IDs (data_type 0: any type);
  * sets OSKAR_ERR_BINARY_TAG_NOT_FOUND if there is none
  * (utils/binary/oskar_binary_query.h). */

@@ -1,0 +1,102 @@
+"""End to end through the example driver fpga_gridding_amd/examples/grid_vis
+(the reference harness's flow, main.cpp:100-605): an OSKAR visibility
+file written by the reference's own binary library (tests/golden/
+ref_vis_f.vis, two blocks), W-kernel cubes in FITS, uniform weights over
+all blocks, one oskar_grid_wproj_f call per block, the grid written as
+FITS.  The grid must meet the FP32 parity bar of the other tests against
+the oracle run on the same blocks in the same order; the skip count must
+match exactly and the norm within 1e-12 relative.
+"""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from fpga_gridding_amd import synth
+from fitsutil import kernel_fits, parse_image
+from parity import assert_fp32_parity, fp32_report
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+EXE = os.path.join(ROOT, "fpga_gridding_amd", "examples", "grid_vis")
+GOLDEN = os.path.join(HERE, "golden")
+C_LIGHT = 299792458.0
+
+def _blocks():
+    d = np.load(os.path.join(GOLDEN, "ref_vis.npz"))
+    s = np.float32(float(d["f_hdr_freq_start_hz"]) / C_LIGHT)
+    out = []
+    for b in range(2):
+        out.append(tuple(d[f"f_b{b}_{k}"] * s for k in ("uu", "vv", "ww"))
+                   + (d[f"f_b{b}_vis"],))
+    return out
+
+
+def _setup(tmp_path, grid_size, planes, s_min, s_max, ov):
+    blocks = _blocks()
+    uu = np.concatenate([b[0] for b in blocks])
+    vv = np.concatenate([b[1] for b in blocks])
+    ww = np.concatenate([b[2] for b in blocks])
+    vis = np.concatenate([b[3] for b in blocks])
+    w_max = float(np.abs(ww.astype(np.float64)).max())
+    support = synth.support_table(planes, s_min, s_max)
+    c = synth.cube_half_size(ov, s_max)
+    cube = synth.wkernel_cube(support, ov, c, w_max)
+    # Most points inside the grid, a few beyond its edge (skipped).
+    r = np.percentile(np.maximum(np.abs(uu), np.abs(vv)), 90)
+    cell = np.float32((grid_size // 2 - s_max - 2) / (r * grid_size))
+    w_scale = np.float32((planes - 1) ** 2 / w_max)
+    root = str(tmp_path / "obs")
+    shutil.copyfile(os.path.join(GOLDEN, "ref_vis_f.vis"), root + ".vis")
+    hdr = dict(oversample=ov, grid_size=grid_size,
+               cell_size="%.9E" % float(cell), w_scale="%.9E" % float(w_scale))
+    kernel_fits(root + "_KERNELS_REAL.fits", cube.real, support, **hdr)
+    kernel_fits(root + "_KERNELS_IMAG.fits", cube.imag, support, **hdr)
+    wl = synth.Workload(
+        name="driver", num_w_planes=planes, support=support, oversample=ov,
+        conv_size_half=c, conv_func=synth.cube_to_conv_func(cube), uu=uu,
+        vv=vv, ww=ww, vis=vis, weight=np.ones(uu.shape[0], np.float32),
+        cell_size_rad=float(cell), w_scale=float(w_scale),
+        grid_size=grid_size)
+    return root, wl
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid_size,planes,s_min,s_max,ov",
+                         [(64, 5, 2, 4, 4), (96, 9, 1, 6, 8)])
+def test_driver_end_to_end(tmp_path, grid_size, planes, s_min, s_max, ov):
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is missing: build it with "
+                    "`python -m fpga_gridding_amd.build`")
+    root, wl = _setup(tmp_path, grid_size, planes, s_min, s_max, ov)
+    out_root = str(tmp_path / "grid")
+    print(f"driver G={grid_size} P={planes} ov={ov}", flush=True)
+    p = subprocess.run([EXE, root, out_root], capture_output=True, text=True,
+                       timeout=60)
+    assert p.returncode == 0, p.stdout + p.stderr
+    m = re.search(r"num_skipped (\d+), norm (\S+),", p.stdout)
+    assert m, p.stdout
+    skipped, norm = int(m.group(1)), float(m.group(2))
+    cards, re_ = parse_image(out_root + "_REAL.fits")
+    _, im = parse_image(out_root + "_IMAG.fits")
+    assert int(cards["NAXIS1"]) == grid_size and int(cards["NAXIS3"]) == 1
+    g = np.empty(2 * grid_size * grid_size, np.float32)
+    g[0::2] = re_.ravel()
+    g[1::2] = im.ravel()
+    r = fp32_report(g, wl)
+    assert 0 < skipped < wl.num_points // 2
+    assert skipped == r["skipped_f"], (skipped, r)
+    assert abs(norm - r["norm_f"]) <= 1e-12 * abs(r["norm_f"]), (norm, r)
+    assert_fp32_parity(r)
+
+
+def test_driver_rejects_missing_input(tmp_path):
+    # Fails before any device call, so it runs without a GPU.
+    if not os.path.exists(EXE):
+        pytest.skip("example driver not built")
+    p = subprocess.run([EXE, str(tmp_path / "none")], capture_output=True,
+                       text=True, timeout=60)
+    assert p.returncode != 0 and "cannot open" in p.stderr

@@ -24,6 +24,8 @@ import numpy as np
 import pytest
 
 from fpga_gridding_amd import io as gio
+from fitsutil import cards as _cards, kernel_fits as _kernel_fits, \
+    pad as _pad, parse_image as _parse_image
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -120,39 +122,6 @@ def test_scale_coords_matches_formula():
 # FITS, written and parsed here from the standard
 # ---------------------------------------------------------------------------
 
-def _cards(pairs):
-    out = b""
-    for k, v in pairs:
-        if k == "END":
-            out += b"END".ljust(80)
-            continue
-        out += (k.ljust(8) + "= " + str(v).rjust(20)).ljust(80).encode()
-    return out.ljust(((len(out) + 2879) // 2880) * 2880, b" ")
-
-
-def _pad(b):
-    return b.ljust(((len(b) + 2879) // 2880) * 2880, b"\0")
-
-
-def _kernel_fits(path, cube, support, bitpix=-32, extra=(), tform="1J"):
-    P, C, _ = cube.shape
-    hdr = [("SIMPLE", "T"), ("BITPIX", bitpix), ("NAXIS", 3),
-           ("NAXIS1", C), ("NAXIS2", C), ("NAXIS3", P), ("EXTEND", "T"),
-           ("OVERSAMP", 4), ("GRIDSIZE", 2048),
-           ("CELLSIZE", "1.7044230976507E-05"), ("W_SCALE", "0.25D+01"),
-           *extra, ("END", None)]
-    fmt = {-32: ">f4", -64: ">f8", 16: ">i2", 32: ">i4"}[bitpix]
-    data = _pad(np.ascontiguousarray(cube).astype(fmt).tobytes())
-    code, width = {"1J": (">i4", 4), "1E": (">f4", 4), "1I": (">i2", 2)}[tform]
-    rows = np.asarray(support).astype(code).tobytes()
-    tab = [("XTENSION", "'BINTABLE'"), ("BITPIX", 8), ("NAXIS", 2),
-           ("NAXIS1", width), ("NAXIS2", len(support)), ("PCOUNT", 0),
-           ("GCOUNT", 1), ("TFIELDS", 1), ("TTYPE1", "'SUPPORT '"),
-           ("TFORM1", f"'{tform}      '"), ("EXTNAME", "'W_KERNELS'"),
-           ("END", None)]
-    open(path, "wb").write(_cards(hdr) + data + _cards(tab) + _pad(rows))
-
-
 def test_read_kernel_float_cube_and_support(tmp_path):
     rng = np.random.default_rng(4)
     cube = rng.normal(size=(3, 9, 9)).astype(np.float32)
@@ -200,26 +169,6 @@ def test_read_kernel_errors(tmp_path):
         gio.read_kernel(path, support=False)         # missing keyword
     with pytest.raises(gio.IOError_):
         gio.read_kernel(str(tmp_path / "none.fits"))
-
-
-def _parse_image(path):
-    """Independent reader: primary header cards and the big-endian image."""
-    raw = open(path, "rb").read()
-    cards, at = {}, 0
-    while True:
-        c = raw[at:at + 80].decode()
-        at += 80
-        if c.startswith("END"):
-            break
-        if c[8:10] == "= ":
-            cards[c[:8].strip()] = c[10:].split("/")[0].strip()
-    data_at = ((at + 2879) // 2880) * 2880
-    bp = int(cards["BITPIX"])
-    shape = tuple(int(cards[f"NAXIS{k}"]) for k in (3, 2, 1))
-    n = int(np.prod(shape))
-    a = np.frombuffer(raw, ">f4" if bp == -32 else ">f8", n, data_at)
-    assert len(raw) % 2880 == 0
-    return cards, a.reshape(shape)
 
 
 def test_write_fits_cube_complex_and_planes(tmp_path):
