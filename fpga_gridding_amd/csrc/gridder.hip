@@ -197,6 +197,7 @@ __global__ void k_init(Summary* __restrict__ s, unsigned* __restrict__ combo,
         s->n_buckets = 0;
         s->total_items = 0;
         s->flags = 0;
+        for (int q = 0; q < 8; ++q) s->item_next[q] = 0;
     }
     for (int i = t; i < n_combo; i += gridDim.x * blockDim.x) combo[i] = 0u;
 }
@@ -539,6 +540,20 @@ __global__ void __launch_bounds__(1024) k_plan(
 //   grid_re += vq.x * c_re + vq.y * c_im,  grid_im += vq.z * c_re + vq.w * c_im
 // with (c_re, c_im) the tap as stored in the kernel cube.  head[s] = 1 where
 // a new tile starts (key bits above tile_shift).
+// meta.y of a point: bits 0-29 the element offset of its (plane, |off_v|,
+// |off_u|) tap table (tables are below 4 GiB and taps at least 4 bytes),
+// bit 30 set for off_u < 0, bit 31 for off_v < 0 (the scatter then reads
+// the table at the mirrored footprint cell).
+__device__ __forceinline__ unsigned table_ref(const PlaneInfo& pi, int off_u,
+                                              int off_v, int h)
+{
+    const int M = h + 1;
+    const unsigned W = (unsigned)pi.W;
+    const unsigned t = (unsigned)(abs(off_v) * M + abs(off_u));
+    return (pi.eoff + t * W * W) | (off_u < 0 ? 1u << 30 : 0u) |
+           (off_v < 0 ? 1u << 31 : 0u);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather(
         const Summary* __restrict__ summary,
@@ -562,10 +577,8 @@ __global__ void __launch_bounds__(256) k_gather(
         const T sgn = ((r.sub >> 16) & 1) ? (T)-1 : (T)1;
         const int off_u = (r.sub & 0xFF) - 128;
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
-        const int table = (off_v + h) * K + (off_u + h);
         const PlaneInfo& pi = pinfo[plane];
-        const int W = pi.W;
-        meta[s] = make_int4(r.pos, (int)(pi.eoff + (unsigned)(table * W * W)),
+        meta[s] = make_int4(r.pos, (int)table_ref(pi, off_u, off_v, h),
                 plane | (pi.srank << 16),
                 gw ? (int)min(gw[i], 0x7FFFFFFFu) : 0);
         typename Types<T>::T4 q;
@@ -789,6 +802,7 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
     const char* __restrict__ ptab = static_cast<const char*>(x.etab);
     const int cstride = LDS ? x.R : x.G;
     unsigned toff[NS];     // byte offset of the lane's tap in a table
+    unsigned k2[NS];       // 2 k sizeof(TapT), k = the lane's footprint column
     unsigned loff[NS];     // LDS: byte offset from the footprint origin in
                            // the region; HBM: cell offset in the grid
     bool valid[NS];
@@ -809,6 +823,7 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
             const unsigned k = cc - j * (unsigned)W;
             loff[s] = j * (unsigned)cstride + k;
             toff[s] = cc * (unsigned)sizeof(TapT);
+            k2[s] = 2 * k * (unsigned)sizeof(TapT);
         }
     }
     else
@@ -823,17 +838,46 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
             const unsigned k = cc - j * (unsigned)W;
             loff[s] = j * (unsigned)cstride + k;
             toff[s] = cc * (unsigned)sizeof(TapT);
+            k2[s] = 2 * k * (unsigned)sizeof(TapT);
         }
     }
+    // Byte offsets in a table of the cell mirrored in u (k -> W-1-k), in v
+    // (j -> W-1-j) and in both, for c = j W + k:
+    //   u: c + (W-1) - 2k,  v: (W-1) W - c + 2k,  both: W^2 - 1 - c.
+    const unsigned bu = (unsigned)(W - 1) * (unsigned)sizeof(TapT);
+    const unsigned bv = (unsigned)((W - 1) * W) * (unsigned)sizeof(TapT);
+    const unsigned bb = (unsigned)(W2 - 1) * (unsigned)sizeof(TapT);
     auto load_taps = [&](int table) {
-        const char* tp = ptab + (size_t)(unsigned)table * sizeof(TapT);
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-        {
-            const auto t = ld_tap(static_cast<const TapT*>(nullptr),
-                    tp + toff[s]);
+        // table = meta.y: element offset in bits 0-29, mirror flags in
+        // bits 30 (u) and 31 (v); the same for every lane.
+        const unsigned e = (unsigned)table & 0x3FFFFFFFu;
+        const unsigned flip = (unsigned)table >> 30;
+        const char* tp = ptab + (size_t)e * sizeof(TapT);
+        auto issue = [&](int s, unsigned o) {
+            const auto t = ld_tap(static_cast<const TapT*>(nullptr), tp + o);
             nre[s] = (T)t.x;
             nim[s] = (T)t.y;
+        };
+        // One branch per point (flip is uniform), not a select per slot.
+        if (flip == 0)
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) issue(s, toff[s]);
+        }
+        else if (flip == 1)
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) issue(s, toff[s] + bu - k2[s]);
+        }
+        else if (flip == 2)
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) issue(s, bv - toff[s] + k2[s]);
+        }
+        else
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) issue(s, bb - toff[s]);
         }
     };
     auto take_taps = [&]() {
@@ -1181,21 +1225,61 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
     }
 }
 
+// Calls f(it) for every scatter item it of [0, s->total_items), each item
+// in exactly one workgroup (persistent grid).  xcds <= 1: grid-stride order.
+// Otherwise: items are in tile Z-order, so a contiguous range of them covers
+// one compact grid region, whose points use a narrower set of w-planes than
+// the whole grid.  Workgroups are dispatched to the XCDs round robin
+// (workgroup b runs on XCD b mod xcds); XCD x takes the x-th of xcds
+// contiguous ranges from its own queue (s->item_next[x], zero at launch), so
+// its L2 holds the tap tables of that region only, then helps the other
+// queues in turn.  Every workgroup must call this (it synchronises).
+template <typename F>
+__device__ __forceinline__ void for_each_item(Summary* __restrict__ s,
+        int xcds, unsigned* s_next, F&& f)
+{
+    const unsigned total = s->total_items;
+    if (xcds <= 1)
+    {
+        for (unsigned it = blockIdx.x; it < total; it += gridDim.x) f(it);
+        return;
+    }
+    const unsigned per = (total + xcds - 1) / xcds;
+    const unsigned home = blockIdx.x % xcds;
+    for (int q = 0; q < xcds; ++q)
+    {
+        const unsigned xq = (home + q) % xcds;
+        const unsigned lo = xq * per;
+        const unsigned hi = min(total, lo + per);
+        for (;;)
+        {
+            if (threadIdx.x == 0) *s_next = lo + atomicAdd(&s->item_next[xq], 1u);
+            __syncthreads();
+            const unsigned it = (unsigned)__builtin_amdgcn_readfirstlane(
+                    (int)*s_next);
+            __syncthreads();
+            if (it >= hi) break;
+            f(it);
+        }
+    }
+}
+
 // Grid-stride loop over the jobs (persistent: the plan launches as many
 // workgroups as fit the device at once).  Taps are stored as TapT pairs;
 // COMP selects the compensated two-float grid layout.
 template <typename T, bool DIAG, typename TapT = typename Types<T>::T2,
           bool COMP = false>
 __global__ void __launch_bounds__(kGridThreads) k_grid(
-        const Summary* __restrict__ summary, const int4* __restrict__ items,
+        Summary* __restrict__ summary, const int4* __restrict__ items,
         const PlaneInfo* __restrict__ pinfo, const int4* __restrict__ meta,
         const typename Types<T>::T4* __restrict__ vq,
         const TapT* __restrict__ etab, int G, int row0,
         int R, T* __restrict__ grid, int force_direct, int min_share,
         unsigned* __restrict__ diag_count,
-        unsigned long long* __restrict__ diag_wsum)
+        unsigned long long* __restrict__ diag_wsum, int xcds)
 {
     __shared__ GridLds L;
+    __shared__ unsigned s_next;
     const int tid = threadIdx.x;
     const int NT = blockDim.x;
     const bool lds = !force_direct && R > 0;
@@ -1216,10 +1300,10 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
         L.re[i] = 0.0;
         L.im[i] = 0.0;
     }
-    const unsigned total = summary->total_items;
-    for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
+    for_each_item(summary, xcds, &s_next, [&](unsigned it) {
         grid_work_item<T, DIAG, TapT, COMP>(L, x, pinfo, items[it], lds,
                                             min_share);
+    });
 }
 
 // -------------------------------------------------------------------------
@@ -1299,26 +1383,33 @@ int bits_for(unsigned long long v)
     return b;
 }
 
-// Expanded tap table: for every plane p and sub-cell offset (off_u, off_v)
-// the (2S+1)^2 taps of one footprint in row-major (j, k) order,
-//   etab[eoff_p + ((off_v+h)*K + off_u+h)*W^2 + (j+S)*W + (k+S)]
-//     = conv_func[p*C^2 + |off_v + j*ov|*C + |off_u + k*ov|]
-// (the reference's flat cube index, oskar_grid_wproj.c:222-233).  Also the
-// per-(plane, sub-cell) tap sums in the reference's order (rows j outer,
-// columns k inner, double accumulation of float taps, :216-238), which are
-// exactly the per-point `sum` the reference adds to the norm.
+// Expanded tap table: for every plane p and sub-cell offset
+// (off_u, off_v) >= 0 the (2S+1)^2 taps of one footprint in row-major (j, k)
+// order,
+//   etab[eoff_p + (off_v*M + off_u)*W^2 + (j+S)*W + (k+S)]
+//     = conv_func[p*C^2 + |off_v + j*ov|*C + |off_u + k*ov|],   M = h + 1
+// (the reference's flat cube index, oskar_grid_wproj.c:222-233).  Negative
+// offsets need no table of their own: |-a + k*ov| = |a + (-k)*ov|, so the
+// tap of (-off_u, off_v) at (j, k) is the tap of (off_u, off_v) at (j, -k),
+// and likewise in v; the scatter reads the mirrored cell (the tables are
+// (h+1)^2 / (2h+1)^2 = 9/25 of the full set at ov = 4).  Also the
+// per-(plane, sub-cell) tap sums for all (2h+1)^2 offsets in the
+// reference's order (rows j outer, columns k inner, double accumulation of
+// float taps, :216-238), which are exactly the per-point `sum` the
+// reference adds to the norm.
 template <typename T>
 void build_tables(const T* conv, size_t P, const int* support, int ov, int C,
                   int h, std::vector<PlaneInfo>& pinfo, std::vector<T>& etab,
                   std::vector<double>& normtab)
 {
     const int K = 2 * h + 1;
+    const int M = h + 1;
     size_t total = 0;
     for (size_t p = 0; p < P; ++p)
     {
         const size_t W = 2 * (size_t)support[p] + 1;
         pinfo[p].eoff = (unsigned)total;
-        total += (size_t)K * K * W * W;
+        total += (size_t)M * M * W * W;
     }
     if (total * 2 * sizeof(T) >= (size_t)4 << 30)
         fail(kErrInvalidArgument,
@@ -1335,8 +1426,9 @@ void build_tables(const T* conv, size_t P, const int* support, int ov, int C,
             for (int iu = 0; iu < K; ++iu)
             {
                 const int off_v = iv - h, off_u = iu - h;
-                T* dst = etab.data() + 2 * (pinfo[p].eoff +
-                        (size_t)(iv * K + iu) * W * W);
+                const bool keep = off_v >= 0 && off_u >= 0;
+                T* dst = keep ? etab.data() + 2 * (pinfo[p].eoff +
+                        (size_t)(off_v * M + off_u) * W * W) : nullptr;
                 double sum = 0.0;
                 for (int j = -S; j <= S; ++j)
                 {
@@ -1345,8 +1437,11 @@ void build_tables(const T* conv, size_t P, const int* support, int ov, int C,
                     for (int k = -S; k <= S; ++k)
                     {
                         const size_t t = 2 * (t1 + std::abs(off_u + k * ov));
-                        *dst++ = conv[t];
-                        *dst++ = conv[t + 1];
+                        if (keep)
+                        {
+                            *dst++ = conv[t];
+                            *dst++ = conv[t + 1];
+                        }
                         sum += conv[t];
                     }
                 }
@@ -1365,14 +1460,14 @@ void build_tables(const T* conv, size_t P, const int* support, int ov, int C,
 // double accumulation of float taps), so both tables are bit-identical to
 // the host build of the same cube.
 __global__ void __launch_bounds__(256) k_expand_taps(
-        const float* __restrict__ conv, int C, int ov, int h, int K,
+        const float* __restrict__ conv, int C, int ov, int M,
         const int* __restrict__ support, const PlaneInfo* __restrict__ pinfo,
         float* __restrict__ etab)
 {
     const int p = blockIdx.y;
     const int S = support[p];
     const int W = 2 * S + 1;
-    const unsigned n = (unsigned)(K * K * W * W);
+    const unsigned n = (unsigned)(M * M * W * W);
     const size_t base = (size_t)p * C * C;
     const size_t e0 = pinfo[p].eoff;
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -1380,10 +1475,10 @@ __global__ void __launch_bounds__(256) k_expand_taps(
     {
         const int combo = (int)(t / (unsigned)(W * W));
         const int jk = (int)(t - (unsigned)combo * W * W);
-        const int iv = combo / K, iu = combo - iv * K;
+        const int iv = combo / M, iu = combo - iv * M;
         const int j = jk / W - S, k = jk - (jk / W) * W - S;
-        const size_t src = 2 * (base + (size_t)abs(iv - h + j * ov) * C +
-                (size_t)abs(iu - h + k * ov));
+        const size_t src = 2 * (base + (size_t)abs(iv + j * ov) * C +
+                (size_t)abs(iu + k * ov));
         etab[2 * (e0 + t)] = conv[src];
         etab[2 * (e0 + t) + 1] = conv[src + 1];
     }
@@ -1613,8 +1708,8 @@ void Plan::build_tables_device(const float* d_conv)
     {
         const size_t W = 2 * (size_t)support_[p] + 1;
         pinfo_[p].eoff = (unsigned)total;
-        total += (size_t)K_ * K_ * W * W;
-        most = std::max(most, (size_t)K_ * K_ * W * W);
+        total += (size_t)(h_ + 1) * (h_ + 1) * W * W;
+        most = std::max(most, (size_t)(h_ + 1) * (h_ + 1) * W * W);
     }
     if (total * 2 * sizeof(float) >= (size_t)4 << 30)
         fail(kErrInvalidArgument,
@@ -1642,7 +1737,7 @@ void Plan::build_tables_device(const float* d_conv)
             hipMemcpyHostToDevice), kErrMemCopy, "upload plane table");
     const unsigned bx = (unsigned)std::min<size_t>((most + 255) / 256, 1024);
     hipLaunchKernelGGL(k_expand_taps, dim3(bx, (unsigned)P_), dim3(256), 0,
-            stream_, d_conv, C_, ov_, h_, K_, (const int*)d_support_,
+            stream_, d_conv, C_, ov_, h_ + 1, (const int*)d_support_,
             (const PlaneInfo*)d_pinfo_, reinterpret_cast<float*>(d_etab_));
     hipLaunchKernelGGL(k_tap_sums, dim3((unsigned)((ntab + 255) / 256)),
             dim3(256), 0, stream_, d_conv, (int)P_, C_, ov_, h_, K_,
@@ -1987,18 +2082,18 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     if (diag)
         hipLaunchKernelGGL((k_grid<T, true>), dim3(grid_blocks_),
                 dim3(kGridThreads), 0, stream_,
-                (const Summary*)d_summary_, (const int4*)d_items_,
+                d_summary_, (const int4*)d_items_,
                 (const PlaneInfo*)d_pinfo_, (const int4*)d_meta_,
                 (const T4*)d_vq_, (const T2*)d_etab_, G_, row0, 0, grid, 1,
-                min_share_, diag->per_cell, diag->wsum_cell);
+                min_share_, diag->per_cell, diag->wsum_cell, xcds_);
     else
         hipLaunchKernelGGL((k_grid<T, false, TapT, COMP>), dim3(grid_blocks_),
                 dim3(kGridThreads), 0, stream_,
-                (const Summary*)d_summary_, (const int4*)d_items_,
+                d_summary_, (const int4*)d_items_,
                 (const PlaneInfo*)d_pinfo_, (const int4*)d_meta_,
                 (const T4*)d_vq_, (const TapT*)d_etab_, G_, row0, region_, grid,
                 0, min_share_, (unsigned*)nullptr,
-                (unsigned long long*)nullptr);
+                (unsigned long long*)nullptr, xcds_);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_grid launch");
     check_hip(hipEventRecord(ev_[4], stream_), kErrKernelLaunch, "event");
 }
@@ -2115,7 +2210,11 @@ __global__ void __launch_bounds__(256) k_batch_key(size_t n,
         unsigned long long sentinel, unsigned long long* __restrict__ out,
         Summary* __restrict__ bsum)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) bsum->nv32 = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        bsum->nv32 = 0;
+        for (int q = 0; q < 8; ++q) bsum->item_next[q] = 0;
+    }
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
     {
@@ -2333,10 +2432,8 @@ __global__ void __launch_bounds__(256) k_gather_batch(
         const float sgn = ((r.sub >> 16) & 1) ? -1.0f : 1.0f;
         const int off_u = (r.sub & 0xFF) - 128;
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
-        const int table = (off_v + d.sub_h) * d.sub_k + (off_u + d.sub_h);
         const PlaneInfo& pi = d.pinfo[plane];
-        const int W = pi.W;
-        meta[s] = make_int4(r.pos, (int)(pi.eoff + (unsigned)(table * W * W)),
+        meta[s] = make_int4(r.pos, (int)table_ref(pi, off_u, off_v, d.sub_h),
                 plane | (pi.srank << 16), 0);
         vq[s] = make_float4(r.re, -sgn * r.im, r.im, sgn * r.re);
     }
@@ -2345,11 +2442,12 @@ __global__ void __launch_bounds__(256) k_gather_batch(
 // The scatter of every job's work items, one item per workgroup at a time
 // (persistent); each item takes its job's tables, grid and region side.
 __global__ void __launch_bounds__(kGridThreads) k_grid_batch(
-        const Summary* __restrict__ bsum, const int4* __restrict__ items,
+        Summary* __restrict__ bsum, const int4* __restrict__ items,
         const BatchJob* __restrict__ jobs, const int4* __restrict__ meta,
-        const float4* __restrict__ vq)
+        const float4* __restrict__ vq, int xcds)
 {
     __shared__ GridLds L;
+    __shared__ unsigned s_next;
     for (int i = threadIdx.x; i < kRegionCells; i += blockDim.x)
     {
         L.re[i] = 0.0;
@@ -2363,9 +2461,7 @@ __global__ void __launch_bounds__(kGridThreads) k_grid_batch(
     x.diag_count = nullptr;
     x.diag_wsum = nullptr;
     x.row0 = 0;
-    const unsigned total = bsum->total_items;
-    for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
-    {
+    for_each_item(bsum, xcds, &s_next, [&](unsigned it) {
         const int4 item = items[it];
         // BatchJob fields by word: pinfo, etab, grid (2 words each), G, R,
         // tile_bits, tile_shift, key_bits, min_share, ...
@@ -2381,7 +2477,7 @@ __global__ void __launch_bounds__(kGridThreads) k_grid_batch(
         const int min_share = (int)w[11];
         grid_work_item<float, false, float2, false>(L, x, pinfo, item,
                 x.R > 0, min_share);
-    }
+    });
 }
 
 Batch::Batch()
@@ -2674,9 +2770,9 @@ void Batch::enqueue()
             (const BatchJob*)d_desc_, (int)J, lmax_, d_meta_,
             (float4*)d_vq_);
     hipLaunchKernelGGL(k_grid_batch, dim3(grid_blocks_), dim3(kGridThreads),
-            0, s, (const Summary*)d_bsum_, (const int4*)d_items_,
+            0, s, d_bsum_, (const int4*)d_items_,
             (const BatchJob*)d_desc_, (const int4*)d_meta_,
-            (const float4*)d_vq_);
+            (const float4*)d_vq_, kXcdGroups);
     check_hip(hipGetLastError(), kErrKernelLaunch, "batch launch");
 }
 

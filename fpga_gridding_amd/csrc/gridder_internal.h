@@ -14,8 +14,9 @@ namespace gw {
 
 // Threads of a scatter workgroup (16 waves), most footprint cells one lane
 // owns at a time (registers per slot: the current taps and the running
-// sums in FP64, the next taps in the plan's precision, complex; 6 slots
-// keep a wave within 128 VGPRs), and the most points in one scatter job
+// sums in FP64, the next taps in the plan's precision, complex; at 6 slots
+// an FP64 wave needs 128 VGPRs and spills a few, and is still faster than
+// at 5, measured), and the most points in one scatter job
 // (the per-job plane segment table lives in LDS).
 constexpr int kGridThreads = 1024;
 constexpr bool kSumF64 = false;
@@ -35,6 +36,9 @@ constexpr int kSinkCells = 64;
 // Default least number of points a scatter stream takes from a plane
 // segment (shorter chunks pay the stream's set-up for too little work).
 constexpr int kMinShare = 8;
+// XCDs of the MI355X (8 per device, 32 CUs each): scatter items are split
+// into this many contiguous ranges, one per XCD.
+constexpr int kXcdGroups = 8;
 
 // Per-plane constants, fixed by the support table (host-computed at plan
 // creation, uploaded once).
@@ -71,6 +75,7 @@ struct Summary
     unsigned total_items;           // work items of the scatter
     unsigned flags;                 // bit 0: diagnostics plane index beyond
                                     // its table; bit 1: work items overflow
+    unsigned item_next[8];          // scatter queue heads, one per XCD
 };
 
 // Per-point record written by the locate kernel in input order.
@@ -310,6 +315,7 @@ private:
     std::vector<int> srank_;
     std::vector<PlaneInfo> pinfo_;
     int min_share_ = kMinShare; // least points per scatter stream
+    int xcds_ = kXcdGroups;     // scatter item queues (1: grid-stride order)
     int slot_cap_ = 8;          // most footprint cells per lane
     int device_ = 0;
     int n_cu_ = 0;
