@@ -546,8 +546,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not "
-              "supported (the gridder runs on a single MI355X; see the "
-              "sharding notes in DESIGN.md)", file=sys.stderr)
+              "supported (the gridder runs on a single MI355X; sharding "
+              "over several GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
 
     import fpga_gridding_amd as fg
@@ -567,7 +567,7 @@ def main():
         print(json.dumps(bench_served(args)))
         return
 
-    wl = synth.config(1)
+    wl = synth.full_scale()
     plan = fg.Plan.for_workload(wl)
     d_uvw = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww)]
     d_vis = fg.DeviceArray.from_host(wl.vis)

@@ -70,7 +70,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
-    # The example driver (host C, links the library in place).
+    # The example program (host C, links the library in place).
     exe = os.path.join(HERE, "examples", "grid_vis")
     cmd = ["gcc", "-O2", "-std=gnu99", "-Wall", "-I", INCLUDE,
            os.path.join(HERE, "examples", "grid_vis.c"), "-o", exe,

@@ -243,16 +243,6 @@ def full_scale(n_st=512, n_times=100, dt_s=10.0, freq_hz=150e6,
                         seed=seed))
 
 
-def config(index: int) -> Workload:
-    """The benchmark workloads by index (0: 64-antenna snapshot,
-    1: 512-station full scale)."""
-    if index == 0:
-        return snapshot_small()
-    if index == 1:
-        return full_scale()
-    raise ValueError(f"no single-job workload with index {index}")
-
-
 def random_small(n=200, grid_size=32, num_planes=3, s_min=1, s_max=3, ov=4,
                  seed=0, edge_fraction=0.1, name="random"):
     """Small random workload for parity tests.

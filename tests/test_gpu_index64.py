@@ -61,7 +61,7 @@ def test_index64_matches_fp64_oracle(name, case):
 
 def test_index64_snapshot64():
     """The 64-antenna snapshot workload (725,760 points, grid 2048^2)."""
-    wl = synth.config(0)
+    wl = synth.snapshot_small()
     err = _check(wl, *fg.grid_workload_index64(wl))
     print(f"index64 snapshot: max_rel vs FP64 oracle {err:.3e}")
 

@@ -127,9 +127,9 @@ def test_plane_index_ties():
     assert_fp32_parity(r)
 
 
-def test_snapshot64_full_config():
+def test_snapshot64_full_workload():
     """The 64-antenna 2 km snapshot (725,760 points, 2048^2, 32 planes)."""
-    wl = synth.config(0)
+    wl = synth.snapshot_small()
     g, sk, nrm = fg.grid_workload_f(wl)
     r = fp32_report(g, wl)
     print({k: v for k, v in r.items() if isinstance(v, float)})
@@ -200,7 +200,7 @@ def test_diagnostics_variant():
 
 def test_resident_plan_matches_dropin():
     """Device-pointer plan API and drop-in call give the same grid."""
-    wl = synth.config(0)
+    wl = synth.snapshot_small()
     g_drop, sk, nrm = fg.grid_workload_f(wl)
     pl = fg.Plan.for_workload(wl)
     d = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww, wl.vis)]
@@ -242,7 +242,7 @@ def test_resident_plan_empty_first_call():
 
 @pytest.fixture(scope="module")
 def full_scale():
-    wl = synth.config(1)
+    wl = synth.full_scale()
     rec, U = oracle.locate_f(wl)
     pl = fg.Plan.for_workload(wl)
     d = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww, wl.vis)]

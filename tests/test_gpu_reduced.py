@@ -77,7 +77,7 @@ def test_reduced_matches_fp64_oracle_on_stored_inputs(prec, name,
                          ids=[p[1] for p in PRECISIONS])
 def test_reduced_snapshot64(prec, name, storage_tol):
     """The 64-antenna snapshot workload (725,760 points, grid 2048^2)."""
-    wl = synth.config(0)
+    wl = synth.snapshot_small()
     g, sk, nrm, _ = fg.grid_workload_reduced(wl, prec)
     g_s, sk_s, n_s = oracle.grid_wproj_d(_stored(wl, prec))
     assert sk == sk_s == 0

@@ -174,7 +174,7 @@ def test_weights_device_api_and_empty():
 def test_weights_full_scale_positions():
     """The 512-station workload's 13,081,600 positions on its 8192^2 grid
     with its weights (uniform in [0.5, 1]): bit-exact write and read."""
-    wl = synth.config(1)
+    wl = synth.full_scale()
     G = wl.grid_size
     g_ref = np.zeros(G * G, np.float32)
     g_dev = np.zeros(G * G, np.float32)

@@ -38,9 +38,10 @@ def _params(wl):
     return wl.support, wl.oversample, wl.conv_size_half, wl.params["w_max"]
 
 
-@pytest.mark.parametrize("which", [0, 1], ids=["snapshot64", "ska512"])
-def test_generated_cube_matches_host_construction(which):
-    wl = synth.config(which)
+@pytest.mark.parametrize("make", [synth.snapshot_small, synth.full_scale],
+                         ids=["snapshot64", "ska512"])
+def test_generated_cube_matches_host_construction(make):
+    wl = make()
     sup, ov, C, w_max = _params(wl)
     dev = fg.wkernel_generate(sup, ov, C, w_max)
     ref = synth.cube_to_conv_func(synth.wkernel_cube(sup, ov, C, w_max))
@@ -48,7 +49,7 @@ def test_generated_cube_matches_host_construction(which):
     d = np.abs(dev.astype(np.float64) - ref.astype(np.float64))
     ulp = np.spacing(np.abs(ref)).astype(np.float64)
     assert np.all(d <= 2 * ulp + 1e-40)
-    print(f"config {which}: {np.count_nonzero(d)} of {d.size} values differ "
+    print(f"{wl.name}: {np.count_nonzero(d)} of {d.size} values differ "
           f"(max {np.max(d / (ulp + 1e-40)):.0f} ulp)")
 
 

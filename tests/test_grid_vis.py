@@ -1,4 +1,4 @@
-"""End to end through the example driver fpga_gridding_amd/examples/grid_vis
+"""End to end through the example program fpga_gridding_amd/examples/grid_vis
 (the reference harness's flow, main.cpp:100-605): an OSKAR visibility
 file written by the reference's own binary library (tests/golden/
 ref_vis_f.vis, two blocks), W-kernel cubes in FITS, uniform weights over
@@ -56,7 +56,7 @@ def _setup(tmp_path, grid_size, planes, s_min, s_max, ov):
     kernel_fits(root + "_KERNELS_REAL.fits", cube.real, support, **hdr)
     kernel_fits(root + "_KERNELS_IMAG.fits", cube.imag, support, **hdr)
     wl = synth.Workload(
-        name="driver", num_w_planes=planes, support=support, oversample=ov,
+        name="grid_vis", num_w_planes=planes, support=support, oversample=ov,
         conv_size_half=c, conv_func=synth.cube_to_conv_func(cube), uu=uu,
         vv=vv, ww=ww, vis=vis, weight=np.ones(uu.shape[0], np.float32),
         cell_size_rad=float(cell), w_scale=float(w_scale),
@@ -67,13 +67,13 @@ def _setup(tmp_path, grid_size, planes, s_min, s_max, ov):
 @pytest.mark.gpu
 @pytest.mark.parametrize("grid_size,planes,s_min,s_max,ov",
                          [(64, 5, 2, 4, 4), (96, 9, 1, 6, 8)])
-def test_driver_end_to_end(tmp_path, grid_size, planes, s_min, s_max, ov):
+def test_grid_vis_end_to_end(tmp_path, grid_size, planes, s_min, s_max, ov):
     if not os.path.exists(EXE):
         pytest.fail(f"{EXE} is missing: build it with "
                     "`python -m fpga_gridding_amd.build`")
     root, wl = _setup(tmp_path, grid_size, planes, s_min, s_max, ov)
     out_root = str(tmp_path / "grid")
-    print(f"driver G={grid_size} P={planes} ov={ov}", flush=True)
+    print(f"grid_vis G={grid_size} P={planes} ov={ov}", flush=True)
     p = subprocess.run([EXE, root, out_root], capture_output=True, text=True,
                        timeout=60)
     assert p.returncode == 0, p.stdout + p.stderr
@@ -93,10 +93,10 @@ def test_driver_end_to_end(tmp_path, grid_size, planes, s_min, s_max, ov):
     assert_fp32_parity(r)
 
 
-def test_driver_rejects_missing_input(tmp_path):
+def test_grid_vis_rejects_missing_input(tmp_path):
     # Fails before any device call, so it runs without a GPU.
     if not os.path.exists(EXE):
-        pytest.skip("example driver not built")
+        pytest.skip("example program not built")
     p = subprocess.run([EXE, str(tmp_path / "none")], capture_output=True,
                        text=True, timeout=60)
     assert p.returncode != 0 and "cannot open" in p.stderr
