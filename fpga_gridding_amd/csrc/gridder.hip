@@ -117,27 +117,26 @@ __device__ __forceinline__ float2 ld_tap(const BF16Pair*, const char* p)
 }
 
 // One component of a compensated two-float grid cell, p[0] = hi and
-// p[1] = lo (value hi + lo): adds x with a 64-bit compare-and-swap that
-// replaces the pair by the double sum hi + lo + x split again into
-// hi = (float)s and lo = (float)(s - hi).
+// p[1] = lo (value hi + lo).  x is split into h = (float)x and
+// l = (float)(x - h); h goes into hi with an atomic add that returns the
+// old value o, whose rounding error e is then exact (Knuth's TwoSum of o
+// and h: the atomic stored fl(o + h), IEEE round to nearest, for operands
+// away from the denormal range), and l + e goes into lo.  Two atomics, one
+// returning, no retries, whatever other adds interleave; the pair is not
+// renormalised (hi + lo is the value).
+__device__ __forceinline__ float two_sum_err(float o, float h)
+{
+    const float s = o + h;
+    const float bp = s - o;
+    return (o - (s - bp)) + (h - bp);
+}
+
 __device__ __forceinline__ void comp_add(float* p, double x)
 {
-    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
-    unsigned long long old = __hip_atomic_load(q, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-    for (;;)
-    {
-        const float hi = __uint_as_float((unsigned)old);
-        const float lo = __uint_as_float((unsigned)(old >> 32));
-        const double sum = (double)hi + (double)lo + x;
-        const float nh = (float)sum;
-        const float nl = (float)(sum - (double)nh);
-        const unsigned long long nv = (unsigned long long)__float_as_uint(nh) |
-                ((unsigned long long)__float_as_uint(nl) << 32);
-        const unsigned long long prev = atomicCAS(q, old, nv);
-        if (prev == old) break;
-        old = prev;
-    }
+    const float h = (float)x;
+    const float l = (float)(x - (double)h);
+    const float o = __builtin_amdgcn_global_atomic_fadd_f32(as_global(p), h);
+    atomic_add_real(p + 1, l + two_sum_err(o, h));
 }
 
 // Adds (re, im) to cell g of the grid: interleaved (re, im) of type T, or
@@ -1029,10 +1028,9 @@ __device__ __forceinline__ void scatter_dispatch(const ScatterCtx<T>& x,
 }
 
 // Region -> compensated grid (re_hi, re_lo, im_hi, im_lo per cell), kBatch
-// cells per thread at a time: the current pairs of all of them are read,
-// then one compare-and-swap each is issued, so the round trips overlap
-// (a pair changed meanwhile by another workgroup is retried alone).
-// Clears the region.  y0, x0: grid row (relative to the rows held) and
+// cells per thread at a time, each component as in comp_add: the returning
+// adds into hi of all of them are issued first, so their round trips
+// overlap, then the adds into lo.  Clears the region.  y0, x0: grid row (relative to the rows held) and
 // column of region cell 0.
 __device__ __forceinline__ void flush_region_comp(GridLds& L, int R, int y0,
         int x0, int G, float* grid)
@@ -1042,9 +1040,8 @@ __device__ __forceinline__ void flush_region_comp(GridLds& L, int R, int y0,
     const int cells = R * R;
     for (int base = threadIdx.x; base < cells; base += NT * kBatch)
     {
-        unsigned long long* q[2 * kBatch];
-        unsigned long long old[2 * kBatch];
-        double add[2 * kBatch];
+        float* q[2 * kBatch];
+        float hi[2 * kBatch], lo[2 * kBatch], old[2 * kBatch];
         bool live[2 * kBatch];
 #pragma unroll
         for (int b = 0; b < kBatch; ++b)
@@ -1058,8 +1055,10 @@ __device__ __forceinline__ void flush_region_comp(GridLds& L, int R, int y0,
             }
             const bool nz = re != 0.0 || im != 0.0;
             live[2 * b] = live[2 * b + 1] = nz;
-            add[2 * b] = re;
-            add[2 * b + 1] = im;
+            hi[2 * b] = (float)re;
+            lo[2 * b] = (float)(re - (double)hi[2 * b]);
+            hi[2 * b + 1] = (float)im;
+            lo[2 * b + 1] = (float)(im - (double)hi[2 * b + 1]);
             q[2 * b] = q[2 * b + 1] = nullptr;
             if (nz)
             {
@@ -1068,37 +1067,20 @@ __device__ __forceinline__ void flush_region_comp(GridLds& L, int R, int y0,
                 const int y = i / R;
                 const int xx = i - y * R;
                 const size_t g = (size_t)((long long)(y0 + y) * G + (x0 + xx));
-                q[2 * b] = reinterpret_cast<unsigned long long*>(grid + 4 * g);
-                q[2 * b + 1] = q[2 * b] + 1;
+                q[2 * b] = grid + 4 * g;
+                q[2 * b + 1] = q[2 * b] + 2;
             }
         }
+        // The returning adds of the batch first, then the low parts (the
+        // same two atomics per component as comp_add, in flight together).
 #pragma unroll
         for (int c = 0; c < 2 * kBatch; ++c)
-            old[c] = live[c] ? __hip_atomic_load(q[c], __ATOMIC_RELAXED,
-                    __HIP_MEMORY_SCOPE_AGENT) : 0ull;
-        auto pack = [](unsigned long long o, double x) {
-            const float hi = __uint_as_float((unsigned)o);
-            const float lo = __uint_as_float((unsigned)(o >> 32));
-            const double sum = (double)hi + (double)lo + x;
-            const float nh = (float)sum;
-            const float nl = (float)(sum - (double)nh);
-            return (unsigned long long)__float_as_uint(nh) |
-                    ((unsigned long long)__float_as_uint(nl) << 32);
-        };
-        unsigned long long prev[2 * kBatch];
+            old[c] = live[c] ? __builtin_amdgcn_global_atomic_fadd_f32(
+                    as_global(q[c]), hi[c]) : 0.0f;
 #pragma unroll
         for (int c = 0; c < 2 * kBatch; ++c)
-            prev[c] = live[c] ? atomicCAS(q[c], old[c], pack(old[c], add[c]))
-                              : old[c];
-#pragma unroll
-        for (int c = 0; c < 2 * kBatch; ++c)
-        {
-            while (prev[c] != old[c])
-            {
-                old[c] = prev[c];
-                prev[c] = atomicCAS(q[c], old[c], pack(old[c], add[c]));
-            }
-        }
+            if (live[c])
+                atomic_add_real(q[c] + 1, lo[c] + two_sum_err(old[c], hi[c]));
     }
 }
 
