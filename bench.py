@@ -391,11 +391,19 @@ def bench_served(args):
     each adds into a resident grid of its kernel set and grid side.  The
     server (gw_server_*) is compared with the plain per-job loop (upload,
     gw_grid_device, gw_plan_results, one request at a time) on the same
-    arrivals."""
+    arrivals.  Request points sit in page-locked host memory (the server's
+    receive buffers; --pageable: ordinary numpy arrays), for both forms."""
     import threading
     import fpga_gridding_amd as fg
     prec = fg.GW_SINGLE if args.index32 else fg.GW_SINGLE_INDEX64
     jobs, sets, plans, dev, tune_s = _mix_setup(prec, autotune=False)
+    if args.pageable:
+        host = [(wl.uu, wl.vv, wl.ww, wl.vis) for wl in jobs]
+        pinned = []
+    else:
+        pinned = [[fg.PinnedArray.copy_of(a) for a in
+                   (wl.uu, wl.vv, wl.ww, wl.vis)] for wl in jobs]
+        host = [tuple(p.array for p in ps) for ps in pinned]
     rng = np.random.default_rng(11)
     R = args.requests
     order = rng.integers(0, len(jobs), R)
@@ -416,7 +424,7 @@ def bench_served(args):
         if wait > 0:
             time.sleep(wait)
         wl = jobs[order[r]]
-        for d, a in zip(stage, (wl.uu, wl.vv, wl.ww, wl.vis)):
+        for d, a in zip(stage, host[order[r]]):
             d.upload(a)
         pl = plans[key[order[r]]]
         pl.grid(wl.num_points, *stage, dev[order[r]][1])
@@ -444,8 +452,8 @@ def bench_served(args):
     def submit(i):
         wl = jobs[i]
         return sv.submit(kid[wl.params["kernel_set"]], wl.grid_size,
-                         wl.cell_size_rad, wl.w_scale, wl.uu, wl.vv, wl.ww,
-                         wl.vis, grid_id=gids[key[i]])
+                         wl.cell_size_rad, wl.w_scale, *host[i],
+                         grid_id=gids[key[i]])
 
     # Warm-up: every job once (plans built and resident).
     for rid in [submit(i) for i in range(len(jobs))]:
@@ -485,6 +493,9 @@ def bench_served(args):
         outs.append(sv.wait(rid)[1])
     accuracy = _vs_fp64_oracle([jobs[j] for j in small], outs)
     sv.close()
+    for ps in pinned:
+        for p in ps:
+            p.close()
     lp = np.array(lat_plain)
     return {
         "metric": "sustained Mvis/s and p50/p99 job latency, served "
@@ -499,7 +510,8 @@ def bench_served(args):
                    "num_points": n_total, "grids": "resident per kernel "
                    "set and grid side", "hbm_budget_gib":
                    args.hbm_budget_gib, "max_batch_jobs": 16,
-                   "streams": 2},
+                   "streams": 2, "request_points": "pageable host memory"
+                   if args.pageable else "page-locked host memory"},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "offered_mvis_s": n_total / arrive[-1] / 1e6,
@@ -538,6 +550,9 @@ def main():
     ap.add_argument("--index32", action="store_true",
                     help="batched / served: FP32 index math (GW_SINGLE) "
                          "instead of FP64 (GW_SINGLE_INDEX64)")
+    ap.add_argument("--pageable", action="store_true",
+                    help="served: request points in pageable host memory "
+                         "instead of page-locked")
     ap.add_argument("--hbm-budget-gib", type=int, default=64,
                     help="served: device-memory budget of the server")
     ap.add_argument("--copy-bytes", type=int, default=0,

@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = (
     "gw_plan_results", "gw_plan_timings", "gw_plan_set_tuning",
     "gw_plan_get_tuning", "gw_plan_autotune",
     "gw_device_count", "gw_device_malloc", "gw_device_free",
+    "gw_host_malloc", "gw_host_free",
     "gw_memcpy_h2d", "gw_memcpy_d2h", "gw_memset_device",
     "gw_device_synchronize", "gw_device_copy", "gw_compensated_resolve",
     "gw_batch_create", "gw_batch_free", "gw_batch_add", "gw_batch_clear",
@@ -152,6 +153,10 @@ def lib():
     L.gw_device_malloc.restype = _vp
     L.gw_device_free.argtypes = [_vp]
     L.gw_device_free.restype = None
+    L.gw_host_malloc.argtypes = [_sz, _ip]
+    L.gw_host_malloc.restype = _vp
+    L.gw_host_free.argtypes = [_vp]
+    L.gw_host_free.restype = None
     for name in ("gw_memcpy_h2d", "gw_memcpy_d2h"):
         getattr(L, name).argtypes = [_vp, _vp, _sz, _ip]
         getattr(L, name).restype = None
@@ -517,6 +522,41 @@ def grid_workload_d(wl, grid=None, norm=0.0):
 # --------------------------------------------------------------------------
 # Device buffers and the resident plan
 # --------------------------------------------------------------------------
+
+class PinnedArray:
+    """Page-locked host memory (gw_host_malloc) seen as a numpy array
+    (``.array``); freed on close/GC.  Copies from it to the device run as
+    DMA, e.g. the points of server requests."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        count = int(np.prod(shape))
+        st = ct.c_int(0)
+        self.ptr = lib().gw_host_malloc(count * dtype.itemsize, ct.byref(st))
+        _check(st, "gw_host_malloc")
+        buf = (ct.c_char * max(1, count * dtype.itemsize)).from_address(
+            self.ptr)
+        self.array = np.frombuffer(buf, dtype=dtype, count=count).reshape(
+            shape)
+
+    @classmethod
+    def copy_of(cls, a: np.ndarray):
+        p = cls(a.shape, a.dtype)
+        p.array[...] = a
+        return p
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            lib().gw_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 class DeviceArray:
     """A device allocation owned by Python (freed on close/GC)."""

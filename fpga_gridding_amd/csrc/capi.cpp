@@ -749,6 +749,24 @@ GW_EXPORT void gw_device_free(void* ptr)
     if (ptr) (void)hipFree(ptr);
 }
 
+GW_EXPORT void* gw_host_malloc(size_t bytes, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return nullptr;
+    void* p = nullptr;
+    guarded(status, [&] {
+        gw::check_hip(hipHostMalloc(&p, bytes ? bytes : 1,
+                hipHostMallocDefault), gw::kErrMemAlloc, "hipHostMalloc");
+    });
+    return p;
+}
+
+GW_EXPORT void gw_host_free(void* ptr)
+{
+    if (ptr) (void)hipHostFree(ptr);
+}
+
 GW_EXPORT void gw_memcpy_h2d(void* dst, const void* src, size_t bytes,
         int* status)
 {

@@ -341,6 +341,11 @@ GW_EXPORT void gw_server_get_stats(gw_server* server, gw_server_stats* out);
 GW_EXPORT int   gw_device_count(void);
 GW_EXPORT void* gw_device_malloc(size_t bytes, int* status);
 GW_EXPORT void  gw_device_free(void* ptr);
+/* Page-locked host memory (hipHostMalloc): copies from it, such as a
+ * server request's points, run as asynchronous DMA instead of staged
+ * copies from pageable memory.  Free with gw_host_free. */
+GW_EXPORT void* gw_host_malloc(size_t bytes, int* status);
+GW_EXPORT void  gw_host_free(void* ptr);
 GW_EXPORT void  gw_memcpy_h2d(void* dst, const void* src, size_t bytes,
         int* status);
 GW_EXPORT void  gw_memcpy_d2h(void* dst, const void* src, size_t bytes,

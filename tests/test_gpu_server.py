@@ -113,6 +113,33 @@ def test_server_same_alone_and_in_mix():
         sv.close()
 
 
+def test_server_pinned_requests_match():
+    """Request points in page-locked host memory (gw_host_malloc), all
+    submitted at once, give each request's result alone from pageable
+    arrays."""
+    jobs, _, sets = _mix(n_jobs=5, seed=24)
+    ref = _alone(jobs, sets)
+    pinned = [[fg.PinnedArray.copy_of(a) for a in
+               (wl.uu, wl.vv, wl.ww, wl.vis)] for wl in jobs]
+    sv = fg.Server(8 * GiB, max_batch_jobs=4, num_streams=2)
+    try:
+        ids = _register(sv, sets)
+        rids = [sv.submit(ids[wl.params["kernel_set"]], wl.grid_size,
+                          wl.cell_size_rad, wl.w_scale,
+                          *(p.array for p in ps))
+                for wl, ps in zip(jobs, pinned)]
+        got = []
+        for rid in rids:
+            res, g = sv.wait(rid, timeout=60.0)
+            got.append((res.num_skipped, res.norm_delta, res.num_updates, g))
+        _check(got, ref)
+    finally:
+        sv.close()
+        for ps in pinned:
+            for p in ps:
+                p.close()
+
+
 def test_server_tight_budget_evicts_and_matches():
     """A budget that holds about one request at a time: plans are evicted
     and re-created, batches shrink, results do not change."""
