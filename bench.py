@@ -439,8 +439,8 @@ def bench_served(args):
     for p in plans.values():
         p.close()
 
-    sv = fg.Server(args.hbm_budget_gib << 30, max_batch_jobs=16,
-                   num_streams=2, precision=prec)
+    sv = fg.Server(args.hbm_budget_gib << 30, max_batch_jobs=args.max_batch,
+                   num_streams=args.streams, precision=prec)
     kid = [sv.add_kernel_set(k.num_w_planes, k.support, k.oversample,
                              k.conv_size_half, k.conv_func) for k in sets]
     gids = {}
@@ -509,8 +509,8 @@ def bench_served(args):
                    "arrivals": f"Poisson, {args.rate} requests/s",
                    "num_points": n_total, "grids": "resident per kernel "
                    "set and grid side", "hbm_budget_gib":
-                   args.hbm_budget_gib, "max_batch_jobs": 16,
-                   "streams": 2, "request_points": "pageable host memory"
+                   args.hbm_budget_gib, "max_batch_jobs": args.max_batch,
+                   "streams": args.streams, "request_points": "pageable host memory"
                    if args.pageable else "page-locked host memory"},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
@@ -550,6 +550,10 @@ def main():
     ap.add_argument("--index32", action="store_true",
                     help="batched / served: FP32 index math (GW_SINGLE) "
                          "instead of FP64 (GW_SINGLE_INDEX64)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="served: server streams (slots)")
+    ap.add_argument("--max-batch", type=int, default=16,
+                    help="served: most requests in one server batch")
     ap.add_argument("--pageable", action="store_true",
                     help="served: request points in pageable host memory "
                          "instead of page-locked")

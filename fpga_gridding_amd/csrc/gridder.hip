@@ -196,7 +196,7 @@ __global__ void k_init(Summary* __restrict__ s, unsigned* __restrict__ combo,
         s->n_buckets = 0;
         s->total_items = 0;
         s->flags = 0;
-        for (int q = 0; q < 8; ++q) s->item_next[q] = 0;
+        for (int q = 0; q < kXcdGroups; ++q) s->item_next[q] = 0;
     }
     for (int i = t; i < n_combo; i += gridDim.x * blockDim.x) combo[i] = 0u;
 }
@@ -624,14 +624,17 @@ __device__ __forceinline__ unsigned compact_bits16(unsigned long long x)
     return (unsigned)x;
 }
 
-// Job = (tile origin u | v << 16, first sorted point, points, 0).
+// Job = (tile origin u | v << 16, first sorted point, points, 0).  Also
+// resets the scatter's queue heads (the kernel runs right before it).
 __global__ void __launch_bounds__(256) k_write_items(
-        const Summary* __restrict__ summary,
+        Summary* __restrict__ summary,
         const unsigned long long* __restrict__ keys,
         const unsigned* __restrict__ bstart,
         const unsigned* __restrict__ ioff, int tile_bits, int tile_shift,
         unsigned job_points, unsigned items_cap, int4* __restrict__ items)
 {
+    if (blockIdx.x == 0 && threadIdx.x < kXcdGroups)
+        summary->item_next[threadIdx.x] = 0;
     const unsigned nb = summary->n_buckets;
     for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b < nb;
          b += gridDim.x * blockDim.x)
@@ -1208,7 +1211,8 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
 }
 
 // Calls f(it) for every scatter item it of [0, s->total_items), each item
-// in exactly one workgroup (persistent grid).  xcds <= 1: grid-stride order.
+// in exactly one workgroup (persistent grid).  xcds <= 1 (or more than the
+// queues in Summary): grid-stride order.
 // Otherwise: items are in tile Z-order, so a contiguous range of them covers
 // one compact grid region, whose points use a narrower set of w-planes than
 // the whole grid.  Workgroups are dispatched to the XCDs round robin
@@ -1221,7 +1225,7 @@ __device__ __forceinline__ void for_each_item(Summary* __restrict__ s,
         int xcds, unsigned* s_next, F&& f)
 {
     const unsigned total = s->total_items;
-    if (xcds <= 1)
+    if (xcds <= 1 || xcds > kXcdGroups)
     {
         for (unsigned it = blockIdx.x; it < total; it += gridDim.x) f(it);
         return;
@@ -2055,7 +2059,7 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     scan_u32(d_head_, d_bpos_, &d_summary_->n_buckets, n,
             &d_summary_->total_items);
     hipLaunchKernelGGL(k_write_items, dim3(blocks), dim3(256), 0, stream_,
-            (const Summary*)d_summary_,
+            d_summary_,
             (const unsigned long long*)d_keys_[1], (const unsigned*)d_bstart_,
             (const unsigned*)d_bpos_, tile_bits_, tile_shift_,
             (unsigned)kJobPoints, (unsigned)items_cap_, d_items_);
@@ -2195,7 +2199,7 @@ __global__ void __launch_bounds__(256) k_batch_key(size_t n,
     if (blockIdx.x == 0 && threadIdx.x == 0)
     {
         bsum->nv32 = 0;
-        for (int q = 0; q < 8; ++q) bsum->item_next[q] = 0;
+        for (int q = 0; q < kXcdGroups; ++q) bsum->item_next[q] = 0;
     }
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)

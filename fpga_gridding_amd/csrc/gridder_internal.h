@@ -75,7 +75,7 @@ struct Summary
     unsigned total_items;           // work items of the scatter
     unsigned flags;                 // bit 0: diagnostics plane index beyond
                                     // its table; bit 1: work items overflow
-    unsigned item_next[8];          // scatter queue heads, one per XCD
+    unsigned item_next[kXcdGroups]; // scatter queue heads, one per XCD
 };
 
 // Per-point record written by the locate kernel in input order.
