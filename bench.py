@@ -151,6 +151,75 @@ def cpu_baseline(wl, budget_s=12.0, seed=3):
                 updates_per_s=done_upd / elapsed, seconds=elapsed)
 
 
+def cpu_baseline_all_cores(wl, seconds=1.5, seed=4, threads=None):
+    """The same reference call on every host core at once (up to 16
+    threads, each gridding its own random 10000-point slices into its own
+    grid; the call releases the GIL), for ``seconds`` of wall time: the
+    host's aggregate rate for independent jobs."""
+    import threading
+    import oracle
+    lib = oracle.ref()
+    kind = "reference"
+    if lib is None:
+        lib = oracle.lib()
+        kind = "port"
+    fn = lib.oskar_grid_wproj_f if kind == "reference" else \
+        lib.gridref_oskar_grid_wproj_f
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    G = wl.grid_size
+    slice_n = 10000
+    support = np.ascontiguousarray(wl.support, np.int32)
+    cube = np.ascontiguousarray(wl.conv_func, np.float32)
+    rng = np.random.default_rng(seed)
+    work = []
+    for _ in range(threads):
+        idx = [np.sort(rng.choice(wl.num_points, slice_n, replace=False))
+               for _ in range(4)]
+        work.append([tuple(np.ascontiguousarray(a[i]) for a in
+                           (wl.uu, wl.vv, wl.ww)) +
+                     (np.ascontiguousarray(wl.vis.reshape(-1, 2)[i]
+                                           .reshape(-1)),) for i in idx])
+    grids = [np.zeros(2 * G * G, np.float32) for _ in range(threads)]
+    counts = [0] * threads
+    stop = time.perf_counter() + seconds
+
+    def run(t):
+        sk = ct.c_size_t(0)
+        nrm = ct.c_double(0.0)
+        k = 0
+        while time.perf_counter() < stop:
+            uu, vv, ww, vis = work[t][k % len(work[t])]
+            fn(wl.num_w_planes, support, wl.oversample, wl.conv_size_half,
+               cube, slice_n, uu, vv, ww, vis, None, wl.cell_size_rad,
+               wl.w_scale, G, ct.byref(sk), ct.byref(nrm), grids[t])
+            counts[t] += slice_n
+            k += 1
+
+    # The reference prints a progress line every 1000 points.
+    libc = ct.CDLL(None)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    null = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(null, 1)
+    try:
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=run, args=(t,)) for t in range(threads)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        wall = time.perf_counter() - t0
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(null)
+    return {"value": sum(counts) / wall / 1e6, "unit": "Mvis/s",
+            "cores": threads, "kind": kind,
+            "sample": f"{sum(counts)} points in {wall:.1f} s wall, "
+                      f"{threads} threads, each its own grid"}
+
+
 def dropin_rate(wl, calls=2):
     """Mvis/s of the drop-in oskar_grid_wproj_f on host arrays (uploads of
     the points, read-modify-write of the touched grid rows over PCIe)."""
@@ -676,6 +745,7 @@ def main():
         out["cpu_baseline"] = {k: cb[k] for k in
                                ("value", "unit", "cores", "kind", "sample")}
         out["cpu_baseline"]["grid_updates_per_s"] = cb["updates_per_s"]
+        out["cpu_baseline"]["all_cores"] = cpu_baseline_all_cores(wl)
         out["accuracy"] = accuracy_sample(wl)
     else:
         out["cpu_baseline"] = None
