@@ -766,16 +766,6 @@ __device__ __forceinline__ void lds_add_cell(GridLds* L, unsigned c8,
     atomicAdd(reinterpret_cast<double*>(b + kImBytes + c8), im);
 }
 
-// Makes the compiler complete the load that produced v before this point:
-// an empty asm that reads and rewrites the register.  It is not volatile
-// and touches no memory, so loads of read-only data around it can still be
-// scalar loads.
-template <typename T>
-__device__ __forceinline__ void settle(T& v)
-{
-    asm("" : "+v"(v));
-}
-
 // One stream's share [a, b) of a plane segment: footprint cells
 // [cb, cb + 64 * wpp * NS), NS cells per lane.  LDS selects the LDS region
 // (else the grid in HBM).
