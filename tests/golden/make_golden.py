@@ -16,6 +16,10 @@ Fixtures:
                    values) repeated to 10000 points.
   ref_d.npz        oskar_grid_wproj_d on the random workload, float64 input.
   ref_weights.npz  oskar_grid_weights_write_f/_read_f and _d.
+  ref_asym.npz     oskar_grid_wproj_f_diagnostics and oskar_grid_wproj_d on
+                   a workload whose kernel cube is random (no symmetry
+                   between rows and columns, or within either), so that a
+                   transposed or mirrored tap index shows.
 
 Usage (where the reference tree is mounted):
   make -C oracle ref && python tests/golden/make_golden.py
@@ -71,6 +75,14 @@ def random_workload():
                               s_max=6, ov=4, seed=11, edge_fraction=0.15)
 
 
+def asymmetric_workload():
+    wl = synth.random_small(n=2000, grid_size=48, num_planes=4, s_min=1,
+                            s_max=5, ov=4, seed=12, edge_fraction=0.1)
+    rng = np.random.default_rng(13)
+    cube = rng.standard_normal(wl.conv_func.shape).astype(np.float32)
+    return dataclasses.replace(wl, conv_func=cube, name="asym")
+
+
 def main():
     ref = oracle.ref()
     if ref is None:
@@ -94,6 +106,14 @@ def main():
     gd, skd, nrmd = oracle.grid_wproj_d(wl, lib_=ref)
     out["ref_d"] = dict(**synth.workload_arrays(wl), grid=gd,
                         num_skipped=skd, norm=nrmd)
+
+    wa = asymmetric_workload()
+    g, sk, nrm, per_s, per_w, per_cell, avg_s = oracle.grid_wproj_f_diagnostics(
+        wa, lib_=ref)
+    gd, skd, nrmd = oracle.grid_wproj_d(wa, lib_=ref)
+    out["ref_asym"] = dict(**synth.workload_arrays(wa), grid=g,
+                           num_skipped=sk, norm=nrm, grid_d=gd,
+                           num_skipped_d=skd, norm_d=nrmd)
 
     G = wl.grid_size
     wgrid = np.zeros(G * G, np.float32)

@@ -69,6 +69,24 @@ def test_golden_fp64_fixture():
     assert max_rel(g, d["grid"]) <= 1e-12
 
 
+def test_golden_asymmetric_cube():
+    """A random kernel cube (no symmetry between or within rows and
+    columns): the scatter's tap tables, including the taps of negative
+    sub-cell offsets read at mirrored cells, against the reference's
+    outputs."""
+    d = _load("ref_asym")
+    wl = synth.workload_from_arrays(d)
+    g, sk, nrm = fg.grid_workload_f(wl)
+    assert sk == int(d["num_skipped"])
+    assert abs(nrm - float(d["norm"])) <= 1e-12 * abs(float(d["norm"]))
+    assert max_rel(g, d["grid"]) <= 1e-5
+    assert_fp32_parity(fp32_report(g, wl))
+    gd, skd, nrmd = fg.grid_workload_d(wl)
+    assert skd == int(d["num_skipped_d"])
+    assert abs(nrmd - float(d["norm_d"])) <= 1e-12 * abs(float(d["norm_d"]))
+    assert max_rel(gd, d["grid_d"]) <= 1e-12
+
+
 # --------------------------------------------------------------------------
 # Shapes against the oracle
 # --------------------------------------------------------------------------

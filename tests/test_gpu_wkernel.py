@@ -15,13 +15,15 @@ Checks:
 * a server kernel set generated on the device gives the results of the
   same set uploaded from the host.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
 import fpga_gridding_amd as fg
 from fpga_gridding_amd import synth
 
-from parity import max_rel
+from parity import assert_fp32_parity, fp32_report, max_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -89,6 +91,44 @@ def test_plan_from_device_cube_grids_as_host_plan():
         (g0, r0), (g1, r1) = out
         assert r0 == r1
         assert max_rel(g0, g1) <= 2e-6
+    finally:
+        for p in plans:
+            p.close()
+        for x in (*d, d_cube):
+            x.close()
+
+
+def test_plan_from_device_cube_asymmetric():
+    """Tables built on the device from a random cube (no symmetry) grid as
+    the host-built tables of the same cube, and the grid meets the FP32
+    parity bar against the oracle."""
+    wl = synth.random_small(n=8000, grid_size=96, num_planes=5, s_min=1,
+                            s_max=7, ov=4, seed=43, edge_fraction=0.05)
+    rng = np.random.default_rng(44)
+    cube = rng.standard_normal(wl.conv_func.shape).astype(np.float32)
+    wl = dataclasses.replace(wl, conv_func=cube)
+    sup, ov, C = wl.support, wl.oversample, wl.conv_size_half
+    d_cube = fg.DeviceArray.from_host(cube)
+    d = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww, wl.vis)]
+    G = wl.grid_size
+    out = []
+    plans = [fg.Plan.from_device_cube(fg.GW_SINGLE, len(sup), sup, ov, C,
+                                      d_cube, wl.cell_size_rad, wl.w_scale,
+                                      G),
+             fg.Plan(fg.GW_SINGLE, len(sup), sup, ov, C, cube,
+                     wl.cell_size_rad, wl.w_scale, G)]
+    try:
+        for pl in plans:
+            g = fg.DeviceArray(8 * G * G)
+            g.zero()
+            pl.grid(wl.num_points, *d, g)
+            out.append((g.download(np.empty(2 * G * G, np.float32)),
+                        pl.results()))
+            g.close()
+        (g0, r0), (g1, r1) = out
+        assert r0 == r1
+        assert max_rel(g0, g1) <= 2e-6
+        assert_fp32_parity(fp32_report(g0, wl))
     finally:
         for p in plans:
             p.close()

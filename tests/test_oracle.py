@@ -76,6 +76,20 @@ def test_golden_d():
     assert _bits_equal(g, d["grid"])
 
 
+def test_golden_asymmetric_cube():
+    """A random kernel cube (no symmetry between or within rows and
+    columns), FP32 and FP64, bit for bit: a transposed or mirrored tap index
+    shows here, where the radially symmetric W-kernels hide it."""
+    d = _load("ref_asym")
+    wl = synth.workload_from_arrays(d)
+    g, sk, nrm = oracle.grid_wproj_f(wl)
+    assert sk == int(d["num_skipped"]) and nrm == float(d["norm"])
+    assert _bits_equal(g, d["grid"])
+    gd, skd, nrmd = oracle.grid_wproj_d(wl)
+    assert skd == int(d["num_skipped_d"]) and nrmd == float(d["norm_d"])
+    assert _bits_equal(gd, d["grid_d"])
+
+
 def test_golden_weights():
     d = _load("ref_weights")
     G = int(d["grid_size"])
