@@ -216,6 +216,27 @@ def test_diagnostics_variant():
     assert max_rel(grid, d["grid"]) <= 1e-5
 
 
+def test_diagnostics_variant_asymmetric_cube():
+    """The diagnostics variant (which adds straight into the grid, not
+    through the LDS region) on the random cube: its grid against the
+    reference's."""
+    d = _load("ref_asym")
+    wl = synth.workload_from_arrays(d)
+    G = wl.grid_size
+    grid = np.zeros(2 * G * G, np.float32)
+    per_s = np.zeros(int(wl.support.max()) + 1, np.int64)
+    per_w = np.zeros(4 * wl.num_w_planes, np.int64)
+    per_cell = np.zeros(G * G, np.int64)
+    avg_s = np.zeros(G * G, np.float32)
+    sk, nrm = fg.grid_wproj_f_diagnostics(
+        wl.num_w_planes, wl.support, wl.oversample, wl.conv_size_half,
+        wl.conv_func, wl.uu, wl.vv, wl.ww, wl.vis, None, wl.cell_size_rad,
+        wl.w_scale, G, grid, per_s, per_w, per_cell, avg_s)
+    assert sk == int(d["num_skipped"])
+    assert abs(nrm - float(d["norm"])) <= 1e-12 * abs(float(d["norm"]))
+    assert max_rel(grid, d["grid"]) <= 1e-5
+
+
 def test_resident_plan_matches_dropin():
     """Device-pointer plan API and drop-in call give the same grid."""
     wl = synth.snapshot_small()
