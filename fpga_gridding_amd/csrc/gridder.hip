@@ -2475,6 +2475,10 @@ Batch::Batch()
         check_hip(hipEventCreate(&e), kErrComputeDevices, "hipEventCreate");
     d_bsum_ = dmalloc<Summary>(1);
     d_ntiles_ = dmalloc<unsigned>(2);
+    check_hip(hipHostMalloc((void**)&h_ntiles_, sizeof(unsigned),
+            hipHostMallocDefault), kErrMemAlloc, "hipHostMalloc");
+    check_hip(hipEventCreateWithFlags(&ev_desc_, hipEventDisableTiming),
+            kErrComputeDevices, "hipEventCreate");
 }
 
 Batch::~Batch()
@@ -2484,6 +2488,9 @@ Batch::~Batch()
     free_buffers();
     dfree(d_bsum_);
     dfree(d_ntiles_);
+    if (h_desc_) (void)hipHostFree(h_desc_);
+    if (h_ntiles_) (void)hipHostFree(h_ntiles_);
+    if (ev_desc_) (void)hipEventDestroy(ev_desc_);
     for (auto& e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
@@ -2491,6 +2498,8 @@ Batch::~Batch()
 
 void Batch::drop_graph()
 {
+    // A replay of the graph may still run on the stream.
+    if (exec_ && stream_) (void)hipStreamSynchronize(stream_);
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     exec_ = nullptr;
@@ -2606,9 +2615,6 @@ void Batch::prepare()
     radix_bits_ = lmax_ + tbits;
     if (radix_bits_ > 64)
         fail(kErrInvalidArgument, "batch key exceeds 64 bits");
-    // The previous run may still read the job table.
-    check_hip(hipStreamSynchronize(stream_), kErrKernelLaunch,
-            "stream synchronize");
     const size_t pcap = std::max<size_t>(pts, 1024);
     items_cap_ = pcap + pcap / kJobPoints + 2;
     // Allocated on first use even for an empty batch: the scans write
@@ -2616,6 +2622,9 @@ void Batch::prepare()
     if (!d_keys_[0] || pts > cap_points_ || tiles > cap_tiles_ ||
             combos > cap_combo_ || J > cap_jobs_)
     {
+        // The previous run may still use the buffers being replaced.
+        check_hip(hipStreamSynchronize(stream_), kErrKernelLaunch,
+                "stream synchronize");
         drop_graph();
         free_buffers();
         cap_points_ = pcap;
@@ -2647,12 +2656,39 @@ void Batch::prepare()
                 hipHostMallocDefault), kErrMemAlloc, "hipHostMalloc");
     }
     items_cap_ = cap_points_ + cap_points_ / kJobPoints + 2;
+    // The job table and tile count go up on the batch's stream, after the
+    // previous run's kernels (which read them) and before this run's: no
+    // host wait for the stream, so uploads queued on it by the caller are
+    // not waited for either.  Only the staging copy of the previous
+    // upload is waited for before it is overwritten.
+    if (desc_pending_)
+    {
+        check_hip(hipEventSynchronize(ev_desc_), kErrKernelLaunch,
+                "event synchronize");
+        desc_pending_ = false;
+    }
+    if (J > h_desc_cap_)
+    {
+        if (h_desc_) (void)hipHostFree(h_desc_);
+        h_desc_ = nullptr;
+        h_desc_cap_ = 0;
+        check_hip(hipHostMalloc((void**)&h_desc_, J * sizeof(BatchJob),
+                hipHostMallocDefault), kErrMemAlloc, "hipHostMalloc");
+        h_desc_cap_ = J;
+    }
     if (J)
-        check_hip(hipMemcpy(d_desc_, desc_.data(), J * sizeof(BatchJob),
-                hipMemcpyHostToDevice), kErrMemCopy, "upload job table");
-    const unsigned nt = (unsigned)tiles;
-    check_hip(hipMemcpy(d_ntiles_, &nt, sizeof(unsigned),
-            hipMemcpyHostToDevice), kErrMemCopy, "upload tile count");
+    {
+        std::memcpy(h_desc_, desc_.data(), J * sizeof(BatchJob));
+        check_hip(hipMemcpyAsync(d_desc_, h_desc_, J * sizeof(BatchJob),
+                hipMemcpyHostToDevice, stream_), kErrMemCopy,
+                "upload job table");
+    }
+    *h_ntiles_ = (unsigned)tiles;
+    check_hip(hipMemcpyAsync(d_ntiles_, h_ntiles_, sizeof(unsigned),
+            hipMemcpyHostToDevice, stream_), kErrMemCopy,
+            "upload tile count");
+    check_hip(hipEventRecord(ev_desc_, stream_), kErrKernelLaunch, "event");
+    desc_pending_ = true;
 }
 
 void Batch::enqueue()

@@ -449,6 +449,14 @@ private:
     unsigned* d_rhist_ = nullptr;     // radix digit counts per chunk
     unsigned* d_roff_ = nullptr;      // their exclusive scan
     unsigned* d_ntiles_ = nullptr;    // [tiles, radix histogram length]
+    // Pinned staging of the job table and tile count, copied on the
+    // batch's stream (stream order keeps the previous run's reads first);
+    // ev_desc_ marks the copy done before the staging is reused.
+    BatchJob* h_desc_ = nullptr;
+    size_t h_desc_cap_ = 0;
+    unsigned* h_ntiles_ = nullptr;
+    hipEvent_t ev_desc_ = nullptr;
+    bool desc_pending_ = false;
     int lmax_ = 0;                    // in-tile key bits of the batch key
     int radix_bits_ = 0;              // significant bits of the batch key
     int4* d_items_ = nullptr;
