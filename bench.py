@@ -598,6 +598,38 @@ def bench_served(args):
     }
 
 
+def _batched_summary(r):
+    """The batched run's figures for the default line."""
+    return {"workload": MIX_WORKLOAD, "num_jobs": r["config"]["num_jobs"],
+            "num_points": r["config"]["num_points"],
+            "value": r["value"], "unit": "Mvis/s",
+            "ms_per_batch": r["ms_per_step"],
+            "graph_device_ms": r["graph_device_ms"],
+            "sequential": r["sequential"],
+            "speedup_vs_sequential": r["speedup_vs_sequential"],
+            "results_identical_to_sequential": r["check"]["results_identical"],
+            "grid_max_rel_vs_sequential": r["check"][
+                "grid_max_rel_4_smallest"],
+            "fp64_oracle": r["accuracy"]["fp64_oracle"],
+            "fp64_oracle_jobs": r["accuracy"]["jobs"],
+            "precision": r["precision"], "autotune_s": r["autotune_s"]}
+
+
+def _served_summary(r):
+    """The served run's figures for the default line."""
+    return {"workload": MIX_WORKLOAD, "requests": r["config"]["requests"],
+            "arrivals": r["config"]["arrivals"],
+            "value": r["value"], "unit": "Mvis/s",
+            "latency_ms": r["latency_ms"],
+            "offered_mvis_s": r["offered_mvis_s"],
+            "plain_loop": r["plain_loop"],
+            "fp64_oracle": r["accuracy"]["fp64_oracle"],
+            "fp64_oracle_jobs": r["accuracy"]["jobs"],
+            "precision": r["precision"],
+            "request_points": r["config"]["request_points"],
+            "hbm_budget_gib": r["config"]["hbm_budget_gib"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -607,6 +639,9 @@ def main():
                     help="skip the CPU baseline and the oracle error sample")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP16 / BF16 plan timings")
+    ap.add_argument("--no-mix", action="store_true",
+                    help="skip the batched and served runs of the 64-job "
+                         "mix in the default (plain) run")
     ap.add_argument("--config", choices=("plain", "batched", "served"),
                     default="plain",
                     help="plain: the 512-station workload (default); "
@@ -727,7 +762,7 @@ def main():
             "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of k_grid<float>, "
                       "copy-calibrated (profiles/k_grid_ska512_traffic.txt), "
                       "over this run's scatter time"},
-        "roofline": {"bound": "mfma", "achieved": achieved,
+        "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic,
@@ -749,6 +784,12 @@ def main():
         out["accuracy"] = accuracy_sample(wl)
     else:
         out["cpu_baseline"] = None
+    if not args.no_mix:
+        plan.close()
+        for x in (*d_uvw, d_vis, d_grid):
+            x.close()
+        out["batched"] = _batched_summary(bench_batched(args))
+        out["served"] = _served_summary(bench_served(args))
     print(json.dumps(out))
 
 

@@ -272,6 +272,16 @@ def from_storage(a, precision):
     raise ValueError("precision must be GW_HALF or GW_BFLOAT16")
 
 
+def resolve_compensated(d_comp, num_cells, d_out):
+    """A compensated two-float grid (re_hi, re_lo, im_hi, im_lo per cell,
+    device) to a float grid (re, im per cell, device); waits."""
+    st = ct.c_int(0)
+    lib().gw_compensated_resolve(d_comp.ptr, num_cells, d_out.ptr, None,
+                                 ct.byref(st))
+    _check(st, "gw_compensated_resolve")
+    device_synchronize()
+
+
 def grid_workload_reduced(wl, precision, comp=None):
     """Grid a synth.Workload with a GW_HALF / GW_BFLOAT16 plan: vis stored
     in 16 bits, float coordinates, compensated device grid.  cell_size_rad
@@ -296,11 +306,7 @@ def grid_workload_reduced(wl, precision, comp=None):
         plan.grid(wl.num_points, *d, d_vis, d_comp)
         sk, nd, _ = plan.results()
         d_out = DeviceArray(8 * g * g)
-        st = ct.c_int(0)
-        lib().gw_compensated_resolve(d_comp.ptr, g * g, d_out.ptr, None,
-                                     ct.byref(st))
-        _check(st, "gw_compensated_resolve")
-        device_synchronize()
+        resolve_compensated(d_comp, g * g, d_out)
         out = d_out.download(np.empty(2 * g * g, np.float32))
         comp_out = d_comp.download(np.empty(4 * g * g, np.float32))
         for x in (*d, d_vis, d_comp, d_out):
@@ -749,6 +755,11 @@ class Batch:
     def clear(self):
         lib().gw_batch_clear(self.ptr)
         self._keep = []
+
+    def set_stream(self, hip_stream):
+        """Run on a caller's HIP stream (an integer handle; work already
+        queued on the previous stream is waited for first)."""
+        lib().gw_batch_set_stream(self.ptr, ct.c_void_p(hip_stream))
 
     def _call(self, name):
         st = ct.c_int(0)

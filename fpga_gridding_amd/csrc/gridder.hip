@@ -1867,13 +1867,13 @@ Plan::~Plan()
 
 void Plan::set_stream(hipStream_t s)
 {
+    if (s == stream_ && s) return;
+    // Calls still queued on the previous stream use the plan's workspace:
+    // finish them first, whoever owns that stream.
+    if (stream_ && (s || !own_stream_)) (void)hipStreamSynchronize(stream_);
     if (s)
     {
-        if (own_stream_ && stream_)
-        {
-            (void)hipStreamSynchronize(stream_);
-            (void)hipStreamDestroy(stream_);
-        }
+        if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
         stream_ = s;
         own_stream_ = false;
     }
@@ -2556,12 +2556,12 @@ void Batch::clear()
 
 void Batch::set_stream(hipStream_t s)
 {
-    if (!s) return;
-    if (own_stream_ && stream_)
-    {
-        (void)hipStreamSynchronize(stream_);
-        (void)hipStreamDestroy(stream_);
-    }
+    if (!s || s == stream_) return;
+    // Work still queued on the previous stream (a run or a graph replay)
+    // reads the job table and the workspace that the next run on s
+    // rewrites: finish it first, whoever owns that stream.
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
     stream_ = s;
     own_stream_ = false;
     dirty_ = true;

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -755,11 +756,14 @@ GW_IO_EXPORT void* gw_read_kernel(const char* file_name, int precision,
     double bscale = 1.0, bzero = 0.0;
     key_double(c, "BSCALE", &bscale);
     key_double(c, "BZERO", &bzero);
-    const size_t n = (size_t)(dims[0] * dims[1] * dims[2]);
     const size_t es = (size_t)std::abs(bp) / 8;
-    if (data_off + n * es > f.size())
+    size_t n = 0, nbytes = 0;
+    if (__builtin_mul_overflow((size_t)dims[0], (size_t)dims[1], &n) ||
+            __builtin_mul_overflow(n, (size_t)dims[2], &n) ||
+            __builtin_mul_overflow(n, es, &nbytes) ||
+            data_off > f.size() || nbytes > f.size() - data_off)
     {
-        *status = kErrFileIO;
+        *status = kErrFileIO;   // the image does not fit the file
         return nullptr;
     }
     const size_t vs = precision == GW_OSKAR_DOUBLE ? 8 : 4;
@@ -827,8 +831,13 @@ GW_IO_EXPORT void* gw_read_kernel(const char* file_name, int precision,
                 double tscal = 1.0, tzero = 0.0;
                 key_double(e, "TSCAL1", &tscal);
                 key_double(e, "TZERO1", &tzero);
-                if (!cb || nx2 < *num_w_planes ||
-                        doff + (size_t)nx1 * nx2 > f.size())
+                // Rows of nx1 bytes, column 1 at the start of each row.
+                size_t last = 0;
+                if (!cb || nx1 < cb || nx2 < *num_w_planes ||
+                        __builtin_mul_overflow((size_t)(*num_w_planes - 1),
+                                               (size_t)nx1, &last) ||
+                        doff > f.size() || last > f.size() - doff ||
+                        (size_t)cb > f.size() - doff - last)
                     break;
                 int* s = static_cast<int*>(std::calloc(*num_w_planes,
                                                        sizeof(int)));
@@ -838,11 +847,24 @@ GW_IO_EXPORT void* gw_read_kernel(const char* file_name, int precision,
                     *status = kErrMemAlloc;
                     return nullptr;
                 }
+                // cfitsio's conversion to TINT without null checking
+                // (getcolk.c): the scaled value, clamped to the int range
+                // (an overflow fails the read), truncated toward zero; NaN
+                // converts as x86 does (INT_MIN).
+                bool overflow = false;
                 for (int r = 0; r < *num_w_planes; ++r)
                 {
                     const double x = be_value(f.data() + doff + (size_t)r * nx1,
                             cbp, 0) * tscal + tzero;
-                    s[r] = (int)std::lround(x);
+                    if (x < -2147483648.49) { overflow = true; s[r] = INT_MIN; }
+                    else if (x > 2147483647.49) { overflow = true; s[r] = INT_MAX; }
+                    else if (std::isnan(x)) s[r] = INT_MIN;
+                    else s[r] = (int)x;
+                }
+                if (overflow)
+                {
+                    std::free(s);
+                    break;
                 }
                 *support = s;
                 found = true;

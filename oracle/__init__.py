@@ -102,6 +102,10 @@ def lib():
             _sz, _i32p, ct.c_int, ct.c_int, _f32p, _sz, _f32p, _f32p, _f32p,
             _f32p, ct.c_float, ct.c_float, ct.c_int, _sz, _i64p, _f64p]
         _lib.gridref_cells_f.restype = None
+        _lib.gridref_cells_d.argtypes = [
+            _sz, _i32p, ct.c_int, ct.c_int, _f64p, _sz, _f64p, _f64p, _f64p,
+            _f64p, ct.c_double, ct.c_double, ct.c_int, _sz, _i64p, _f64p]
+        _lib.gridref_cells_d.restype = None
         _lib.gridref_oskar_grid_wproj_f_exact.argtypes = [
             _sz, _i32p, ct.c_int, ct.c_int, _f32p, _sz, _f32p, _f32p, _f32p,
             _f32p, ct.c_float, ct.c_float, ct.c_int, _szp, _dp, _f64p]
@@ -233,6 +237,42 @@ def locate_f(wl):
     rec = {k: out[:, i].copy() for i, k in enumerate(names)}
     rec["grid_w"] = gw
     return rec, int(u)
+
+
+def locate_d(wl):
+    """Per-point FP64 index stage (oskar_grid_wproj.c:71-101) on the
+    float32 coordinates widened to double, with the cell size and w scale
+    as float32 values: dict of arrays + U."""
+    n = wl.num_points
+    out = np.zeros((n, 8), np.int32)
+    gw = np.zeros(n, np.uint64)
+    u = lib().gridref_locate_d(
+        wl.num_w_planes, _c(wl.support, np.int32), wl.oversample, n,
+        _c(wl.uu, np.float64), _c(wl.vv, np.float64), _c(wl.ww, np.float64),
+        float(np.float32(wl.cell_size_rad)), float(np.float32(wl.w_scale)),
+        wl.grid_size, out.ctypes.data_as(ct.c_void_p),
+        gw.ctypes.data_as(ct.c_void_p))
+    names = ["grid_u", "grid_v", "off_u", "off_v", "plane", "support",
+             "skip", "conj"]
+    rec = {k: out[:, i].copy() for i, k in enumerate(names)}
+    rec["grid_w"] = gw
+    return rec, int(u)
+
+
+def cells_d(wl, cells):
+    """float64 values of the given grid cells (v*G + u) from all points with
+    the FP64 oracle's index math and arithmetic (inputs widened from the
+    workload's float32 arrays, cell size and w scale as float32 values)."""
+    cells = _c(cells, np.int64)
+    out = np.zeros(2 * cells.shape[0], np.float64)
+    lib().gridref_cells_d(
+        wl.num_w_planes, _c(wl.support, np.int32), wl.oversample,
+        wl.conv_size_half, _c(wl.conv_func, np.float64), wl.num_points,
+        _c(wl.uu, np.float64), _c(wl.vv, np.float64), _c(wl.ww, np.float64),
+        _c(wl.vis, np.float64), float(np.float32(wl.cell_size_rad)),
+        float(np.float32(wl.w_scale)), wl.grid_size, cells.shape[0], cells,
+        out)
+    return out
 
 
 def cells_f(wl, cells):

@@ -21,7 +21,7 @@
  * fixed to 1 (:185), double `sum` of float taps (:165, :238), footprint
  * rows j outer and columns k inner (:216-240), `norm += sum * 1.0f` (:241).
  *
- * Extra entry points (gridref_locate_f/_d, gridref_cells_f) expose the
+ * Extra entry points (gridref_locate_f/_d, gridref_cells_f/_d) expose the
  * per-point index stage and single-cell brute force for tests.
  */
 #include <math.h>
@@ -412,6 +412,49 @@ EXPORT void gridref_cells_f(const size_t num_w_planes, const int* support,
                     (double)vis[2 * i + 1] * c_im;
             out[2 * c + 1] += (double)vis[2 * i + 1] * c_re +
                     (double)vis[2 * i] * c_im;
+        }
+    }
+}
+
+/* The same brute force with the FP64 index math and arithmetic of
+ * oskar_grid_wproj_d (oskar_grid_wproj.c:71-112): double coordinates,
+ * cube and visibilities.  Used to spot-check the FP64-index plans (the
+ * reduced-precision ones on their stored inputs) on large grids. */
+EXPORT void gridref_cells_d(const size_t num_w_planes, const int* support,
+        const int oversample, const int conv_size_half,
+        const double* conv_func, const size_t num_points, const double* uu,
+        const double* vv, const double* ww, const double* vis,
+        const double cell_size_rad, const double w_scale, const int grid_size,
+        const size_t num_cells, const long long* cells, double* out)
+{
+    const double grid_scale = grid_size * cell_size_rad;
+    const size_t kdim = (size_t)conv_size_half * conv_size_half;
+    size_t i, c;
+    memset(out, 0, 2 * num_cells * sizeof(double));
+    for (i = 0; i < num_points; ++i)
+    {
+        locate_t r;
+        locate_d(uu[i], vv[i], ww[i], grid_scale, w_scale, oversample,
+                grid_size, num_w_planes, support, &r);
+        if (r.skip) continue;
+        for (c = 0; c < num_cells; ++c)
+        {
+            const long long cu = cells[c] % grid_size;
+            const long long cv = cells[c] / grid_size;
+            const long long k = cu - r.grid_u, j = cv - r.grid_v;
+            size_t t;
+            double c_re, c_im;
+            if (k < -r.support || k > r.support ||
+                    j < -r.support || j > r.support)
+                continue;
+            t = (r.plane * kdim +
+                    (size_t)abs(r.off_v + (int)j * oversample) *
+                    conv_size_half +
+                    (size_t)abs(r.off_u + (int)k * oversample)) << 1;
+            c_re = conv_func[t];
+            c_im = conv_func[t + 1] * (r.conj ? -1.0 : 1.0);
+            out[2 * c]     += vis[2 * i] * c_re - vis[2 * i + 1] * c_im;
+            out[2 * c + 1] += vis[2 * i + 1] * c_re + vis[2 * i] * c_im;
         }
     }
 }

@@ -85,11 +85,13 @@ def tap_tables(wl):
     return re, cx, h
 
 
-def expected_sums(wl, rec):
+def expected_sums(wl, rec, tabs=None):
     """Closed-form norm increment and complex grid checksum (sum over all
     cells) from the per-point index results; also the scale
-    sum_i |v_i| * sum_footprint |taps| that bounds the checksum's error."""
-    re, cx, h = tap_tables(wl)
+    sum_i |v_i| * sum_footprint |taps| that bounds the checksum's error.
+    ``tabs``: tap_tables(wl) when already computed (workloads sharing a
+    kernel set)."""
+    re, cx, h = tap_tables(wl) if tabs is None else tabs
     ok = rec["skip"] == 0
     p = rec["plane"][ok]
     iv = rec["off_v"][ok] + h

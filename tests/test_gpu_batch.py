@@ -15,6 +15,7 @@ program's own work (grid clearing, result reads) between replays.
 import numpy as np
 import pytest
 
+import oracle
 import fpga_gridding_amd as fg
 from fpga_gridding_amd import synth
 
@@ -100,6 +101,18 @@ def _check(batch, jobs, ref):
             assert max_rel(g, g_ref) <= 2e-6, j
         else:
             assert not np.any(g), j
+    # The smallest job with points against the oracle directly (skip count,
+    # norm and grid to the FP32 parity bar), not only against the plain
+    # path the batch shares kernels with.
+    live = [j for j in range(len(jobs.jobs)) if jobs.jobs[j].num_points]
+    if live:
+        j = min(live, key=lambda i: jobs.jobs[i].num_points)
+        g_e, sk_e, n_e = oracle.grid_wproj_f_exact(jobs.jobs[j])
+        sk, nd, _ = batch.results(j)
+        assert sk == sk_e
+        assert abs(nd - n_e) <= 1e-12 * max(abs(n_e), 1.0)
+        if np.any(g_e):
+            assert max_rel(jobs.grid(j), g_e) <= 1e-5
 
 
 def test_batch_matches_sequential_jobs():
@@ -215,3 +228,87 @@ def test_autotune_keeps_results():
         for x in (*d, g, scratch):
             x.close()
         pl.close()
+
+
+class _HipStreams:
+    """Caller-owned HIP streams (test infrastructure: the HIP runtime the
+    library uses, through ctypes)."""
+
+    def __init__(self, n):
+        import ctypes as ct
+        self.hip = ct.CDLL("libamdhip64.so")
+        self.streams = []
+        for _ in range(n):
+            s = ct.c_void_p()
+            assert self.hip.hipStreamCreate(ct.byref(s)) == 0
+            self.streams.append(s.value)
+
+    def close(self):
+        import ctypes as ct
+        for s in self.streams:
+            self.hip.hipStreamSynchronize(ct.c_void_p(s))
+            self.hip.hipStreamDestroy(ct.c_void_p(s))
+        self.streams = []
+
+
+def test_batch_reuse_back_to_back():
+    """One Batch, job lists changed between runs with nothing waited for in
+    between: a run within capacity, a run that grows the buffers, a graph
+    capture and replay, then a job change and a replay (recapture while the
+    previous replay may still run) on another caller stream.  Every round
+    grids into its own grids; all are checked at the end against the jobs
+    gridded alone and against the oracle."""
+    rounds = [_mix(n_jobs=3, seed=21),
+              _mix(n_jobs=3, seed=22),
+              _mix(n_jobs=5, seed=23, n_range=(60000, 150000)),
+              _mix(n_jobs=4, seed=24),
+              _mix(n_jobs=3, seed=25)]
+    sets = [_Jobs(m[0]) for m in rounds]
+    streams = _HipStreams(2)
+    try:
+        refs = [_sequential(js) for js in sets]
+        b = fg.Batch()
+
+        def add_all(js):
+            b.clear()
+            for j, wl in enumerate(js.jobs):
+                d, g = js.dev[j]
+                b.add(js.plan(j), wl.num_points, *d, g)
+
+        add_all(sets[0])
+        b.run()                       # round 0: first allocation
+        add_all(sets[1])
+        b.run()                       # round 1: same capacity
+        add_all(sets[2])
+        b.run()                       # round 2: buffers grow
+        b.set_stream(streams.streams[0])
+        add_all(sets[3])
+        b.capture()
+        b.launch()                    # round 3: graph on a caller stream
+        b.set_stream(streams.streams[1])
+        add_all(sets[4])
+        b.launch()                    # round 4: jobs changed, recaptured
+        for j in range(len(sets[4].jobs)):
+            sk, nd, nu = b.results(j)
+            assert (sk, nu) == (refs[4][j][1][0], refs[4][j][1][2])
+        fg.device_synchronize()
+        for r, js in enumerate(sets):
+            for j, wl in enumerate(js.jobs):
+                g_ref = refs[r][j][0]
+                g = js.grid(j)
+                if np.any(g_ref):
+                    assert max_rel(g, g_ref) <= 2e-6, (r, j)
+                else:
+                    assert not np.any(g), (r, j)
+        # The oracle on the smallest job of each round (FP32 index math,
+        # FP64 sums: the batch's FP32 grid to the FP32 parity bar).
+        for r, js in enumerate(sets):
+            j = min(range(len(js.jobs)), key=lambda i: js.jobs[i].num_points)
+            g_e, sk_e, _ = oracle.grid_wproj_f_exact(js.jobs[j])
+            assert refs[r][j][1][0] == sk_e
+            assert max_rel(js.grid(j), g_e) <= 1e-5, (r, j)
+        b.close()
+    finally:
+        streams.close()
+        for js in sets:
+            js.close()

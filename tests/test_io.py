@@ -152,6 +152,36 @@ def test_read_kernel_conversions(tmp_path):
     assert _bits(got, ints * 0.5 + 1.0) and list(s) == [3, 4]
 
 
+def test_read_kernel_support_truncates_and_bounds(tmp_path):
+    """A float support column converts as cfitsio's TINT read does
+    (truncation toward zero, getcolk.c); a table whose rows are narrower
+    than the column, or that the file cuts short, is not read."""
+    cube = np.zeros((3, 5, 5), np.float32)
+    path = str(tmp_path / "e.fits")
+    _kernel_fits(path, cube, [1.7, 2.99, -1.6], tform="1E")
+    _, _, s = gio.read_kernel(path)
+    assert list(s) == [1, 2, -1]
+    raw = open(path, "rb").read()
+    # Row width 0: the column does not fit a row.
+    bad = raw.replace(b"NAXIS1  =                    4",
+                      b"NAXIS1  =                    0")
+    assert bad != raw
+    open(path, "wb").write(bad)
+    with pytest.raises(gio.IOError_):
+        gio.read_kernel(path)
+    # Rows past the end of the file.
+    long_rows = raw.replace(b"NAXIS2  =                    3",
+                            b"NAXIS2  =                 9999")
+    assert long_rows != raw
+    open(path, "wb").write(long_rows)
+    gio.read_kernel(path)                            # rows 1..3 are there
+    big = raw.replace(b"NAXIS1  =                    4",
+                      b"NAXIS1  =              1000000")
+    open(path, "wb").write(big)
+    with pytest.raises(gio.IOError_):
+        gio.read_kernel(path)
+
+
 def test_read_kernel_errors(tmp_path):
     cube = np.zeros((2, 5, 5), np.float32)
     path = str(tmp_path / "nosup.fits")

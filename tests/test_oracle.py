@@ -104,6 +104,19 @@ def test_golden_weights():
                                     G, wg)
     assert sk == int(d["skipped_read_f"])
     assert _bits_equal(wo, d["weight_out_f"])
+    # FP64 (utils/oskar_grid_weights.c:38-104): the same points widened to
+    # double, the cell size as the float the fixture stores.
+    uu_d = d["uu"].astype(np.float64)
+    vv_d = d["vv"].astype(np.float64)
+    w_d = d["weight"].astype(np.float64)
+    wg_d = np.zeros(G * G, np.float64)
+    sk = oracle.grid_weights_write_d(uu_d, vv_d, w_d, cell, G, wg_d)
+    assert sk == int(d["skipped_write_d"])
+    assert _bits_equal(wg_d, d["grid_d"])
+    wo_d = np.zeros(d["uu"].size, np.float64)
+    sk = oracle.grid_weights_read_d(uu_d, vv_d, w_d, wo_d, cell, G, wg_d)
+    assert sk == int(d["skipped_read_d"])
+    assert _bits_equal(wo_d, d["weight_out_d"])
 
 
 # --------------------------------------------------------------------------
@@ -312,6 +325,26 @@ def test_cells_bruteforce_matches_scatter():
     cells = np.arange(wl.grid_size ** 2, dtype=np.int64)
     out = oracle.cells_f(wl, cells)
     np.testing.assert_allclose(out, gd, rtol=0, atol=1e-12 * np.abs(gd).max())
+
+
+def test_cells_fp64_bruteforce_matches_fp64_scatter():
+    """gridref_cells_d (per-cell brute force with the FP64 index math)
+    equals gridref_oskar_grid_wproj_d on every cell, including the
+    edge-case points (ties, w = +-0, grid edges) where the FP32 index math
+    picks other cells or sub-cell tables -- there the FP32 brute force
+    differs, so an FP32 index in cells_d would fail here."""
+    for wl in (synth.random_small(n=3000, grid_size=64, num_planes=5,
+                                  s_min=1, s_max=5, seed=31,
+                                  edge_fraction=0.2),
+               synth.edge_cases()):
+        gd, skd, _ = oracle.grid_wproj_d(wl)
+        rec, _ = oracle.locate_d(wl)
+        assert skd == int(np.sum(rec["skip"] != 0))
+        cells = np.arange(wl.grid_size ** 2, dtype=np.int64)
+        scale = np.abs(gd).max()
+        np.testing.assert_allclose(oracle.cells_d(wl, cells), gd, rtol=0,
+                                   atol=1e-12 * scale)
+    assert np.abs(oracle.cells_f(wl, cells) - gd).max() > 1e-3 * scale
 
 
 def test_exact_fp32_index_oracle():
