@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "oskar_grid_wproj_last_error", "gw_error_string", "gw_last_error_detail",
     "gw_release_cached",
     "gw_plan_create", "gw_plan_free", "gw_plan_set_stream", "gw_grid_device",
+    "gw_grid_device_noalloc",
     "gw_plan_results", "gw_plan_timings", "gw_plan_set_tuning",
     "gw_plan_get_tuning", "gw_plan_autotune",
     "gw_device_count", "gw_device_malloc", "gw_device_free",
@@ -135,6 +136,9 @@ def lib():
     L.gw_plan_set_stream.restype = None
     L.gw_grid_device.argtypes = [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _ip]
     L.gw_grid_device.restype = None
+    L.gw_grid_device_noalloc.argtypes = [_vp, _sz, _vp, _vp, _vp, _vp, _vp,
+                                         _ip]
+    L.gw_grid_device_noalloc.restype = None
     L.gw_plan_results.argtypes = [_vp, _szp, _dp,
                                   ct.POINTER(ct.c_ulonglong), _ip]
     L.gw_plan_results.restype = None
@@ -680,6 +684,15 @@ class Plan:
         lib().gw_grid_device(self.ptr, n, _ptr(d_uu), _ptr(d_vv), _ptr(d_ww),
                              _ptr(d_vis), _ptr(d_grid), ct.byref(st))
         _check(st, "gw_grid_device")
+
+    def grid_noalloc(self, n, d_uu, d_vv, d_ww, d_vis, d_grid):
+        """gw_grid_device_noalloc: as grid(), with the per-call workspace in
+        the library's static device scratch (nothing allocated)."""
+        st = ct.c_int(0)
+        lib().gw_grid_device_noalloc(self.ptr, n, _ptr(d_uu), _ptr(d_vv),
+                                     _ptr(d_ww), _ptr(d_vis), _ptr(d_grid),
+                                     ct.byref(st))
+        _check(st, "gw_grid_device_noalloc")
 
     def results(self):
         st = ct.c_int(0)

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -651,6 +652,98 @@ void gw_grid_device(gw_plan* plan, size_t num_points, const void* d_uu,
             gw::fail(gw::kErrInvalidArgument, "device pointers must not be NULL");
         plan->impl->locate(num_points, d_uu, d_vv, d_ww, d_vis, nullptr);
         plan->impl->scatter(num_points, d_grid, 0, nullptr);
+    });
+}
+
+// -------------------------------------------------------------------------
+// Allocation-free gridding call
+// -------------------------------------------------------------------------
+
+namespace {
+
+// The per-call workspace of gw_grid_device_noalloc (sort keys and values,
+// point records, tile tables, work items, sort temporaries): one copy of
+// this array per GPU the library runs on, held until the process ends.
+// In host code its name is a stand-in for the symbol; the device address
+// comes from hipGetSymbolAddress (once per device) and only that address is
+// passed to kernels.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (looked up with that device
+// current on its first use, the caller's current device restored).
+void* scratch_on(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        gw::fail(gw::kErrComputeDevices, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device])
+    {
+        int cur = 0;
+        gw::check_hip(hipGetDevice(&cur), gw::kErrComputeDevices,
+                      "hipGetDevice");
+        if (cur != device)
+            gw::check_hip(hipSetDevice(device), gw::kErrComputeDevices,
+                          "hipSetDevice");
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        gw::check_hip(e, gw::kErrComputeDevices, "hipGetSymbolAddress");
+        if (!p)
+            gw::fail(gw::kErrComputeDevices, "scratch has no device address");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+// Most points of one gw_grid_device_noalloc call with this plan.
+size_t noalloc_max_points(const gw::Plan& p)
+{
+    size_t lo = 0, hi = kNoAllocScratchBytes;
+    while (lo < hi)
+    {
+        const size_t mid = lo + (hi - lo + 1) / 2;
+        if (p.external_workspace_bytes(mid) <= kNoAllocScratchBytes) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace
+
+GW_EXPORT
+void gw_grid_device_noalloc(gw_plan* plan, size_t num_points,
+        const void* d_uu, const void* d_vv, const void* d_ww,
+        const void* d_vis, void* d_grid, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!plan) gw::fail(gw::kErrInvalidArgument, "plan is NULL");
+        if (!d_grid || (num_points && (!d_uu || !d_vv || !d_ww || !d_vis)))
+            gw::fail(gw::kErrInvalidArgument, "device pointers must not be NULL");
+        gw::Plan& p = *plan->impl;
+        if (p.external_workspace_bytes(num_points) > kNoAllocScratchBytes)
+            gw::fail(gw::kErrOutOfRange, "gw_grid_device_noalloc: " +
+                     std::to_string(num_points) + " points exceed its " +
+                     std::to_string(kNoAllocScratchBytes >> 20) +
+                     " MiB scratch (at most " +
+                     std::to_string(noalloc_max_points(p)) +
+                     " points per call with this plan)");
+        void* base = scratch_on(p.device());
+        struct Restore
+        {
+            gw::Plan& p;
+            ~Restore() { p.restore_workspace(); }
+        } restore{p};
+        p.use_external_workspace(base, kNoAllocScratchBytes, num_points);
+        p.locate(num_points, d_uu, d_vv, d_ww, d_vis, nullptr);
+        p.scatter(num_points, d_grid, 0, nullptr);
     });
 }
 

@@ -1885,6 +1885,138 @@ void Plan::set_stream(hipStream_t s)
     }
 }
 
+Plan::Workspace Plan::save_workspace() const
+{
+    Workspace w;
+    w.cap = cap_;
+    w.items_cap = items_cap_;
+    for (int i = 0; i < 2; ++i)
+    {
+        w.keys[i] = d_keys_[i];
+        w.idx[i] = d_idx_[i];
+    }
+    w.rec = d_rec_;
+    w.meta = d_meta_;
+    w.vq = d_vq_;
+    w.gw = d_gw_;
+    w.sort_tmp = d_sort_tmp_;
+    w.sort_tmp_bytes = sort_tmp_bytes_;
+    w.head = d_head_;
+    w.bpos = d_bpos_;
+    w.bstart = d_bstart_;
+    w.scan_part = d_scan_part_;
+    w.items = d_items_;
+    return w;
+}
+
+void Plan::set_workspace(const Workspace& w)
+{
+    cap_ = w.cap;
+    items_cap_ = w.items_cap;
+    for (int i = 0; i < 2; ++i)
+    {
+        d_keys_[i] = w.keys[i];
+        d_idx_[i] = w.idx[i];
+    }
+    d_rec_ = w.rec;
+    d_meta_ = w.meta;
+    d_vq_ = w.vq;
+    d_gw_ = w.gw;
+    d_sort_tmp_ = w.sort_tmp;
+    sort_tmp_bytes_ = w.sort_tmp_bytes;
+    d_head_ = w.head;
+    d_bpos_ = w.bpos;
+    d_bstart_ = w.bstart;
+    d_scan_part_ = w.scan_part;
+    d_items_ = w.items;
+}
+
+// Carves a workspace for cap points (the layout of ensure_capacity) from
+// [base, base + bytes) with 256-byte aligned parts; with base == nullptr
+// only counts.  Returns the bytes used.
+static size_t carve_workspace(char* base, size_t cap, size_t rec_bytes,
+        size_t real_size, size_t sort_tmp_bytes, size_t items_cap,
+        unsigned long long** keys, unsigned** idx, void** rec, int4** meta,
+        void** vq, unsigned** gw, void** sort_tmp, unsigned** head,
+        unsigned** bpos, unsigned** bstart, unsigned** scan_part,
+        int4** items)
+{
+    size_t off = 0;
+    auto take = [&](size_t n) -> char* {
+        char* p = base ? base + off : nullptr;
+        off += (n + 255) & ~(size_t)255;
+        return p;
+    };
+    for (int i = 0; i < 2; ++i)
+    {
+        keys[i] = reinterpret_cast<unsigned long long*>(take(cap * 8));
+        idx[i] = reinterpret_cast<unsigned*>(take(cap * 4));
+    }
+    *rec = take(cap * rec_bytes);
+    *meta = reinterpret_cast<int4*>(take(cap * sizeof(int4)));
+    *vq = take(cap * 4 * real_size);
+    *gw = reinterpret_cast<unsigned*>(take(cap * 4));
+    *head = reinterpret_cast<unsigned*>(take(cap * 4));
+    *bpos = reinterpret_cast<unsigned*>(take(cap * 4));
+    *bstart = reinterpret_cast<unsigned*>(take((cap + 1) * 4));
+    *scan_part = reinterpret_cast<unsigned*>(take((cap / kScanTile + 2) * 4));
+    *sort_tmp = take(sort_tmp_bytes);
+    *items = reinterpret_cast<int4*>(take(items_cap * sizeof(int4)));
+    return off;
+}
+
+size_t Plan::external_workspace_bytes(size_t n) const
+{
+    const size_t cap = std::max<size_t>(n, 1024);
+    size_t sort_bytes = 0;
+    check_hip(rocprim::radix_sort_pairs(nullptr, sort_bytes,
+            (unsigned long long*)nullptr, (unsigned long long*)nullptr,
+            (unsigned*)nullptr, (unsigned*)nullptr, cap, 0u,
+            (unsigned)key_bits_ + 1u, stream_), kErrComputeDevices,
+            "radix sort size query");
+    Workspace w{};
+    return carve_workspace(nullptr, cap, prec_ == kPrecDouble ?
+            sizeof(RecD) : sizeof(RecF), prec_ == kPrecDouble ?
+            sizeof(double) : sizeof(float), sort_bytes,
+            cap + cap / kJobPoints + 2, w.keys, w.idx, &w.rec, &w.meta,
+            &w.vq, &w.gw, &w.sort_tmp, &w.head, &w.bpos, &w.bstart,
+            &w.scan_part, &w.items);
+}
+
+void Plan::use_external_workspace(void* base, size_t bytes, size_t n)
+{
+    if (external_) fail(kErrInvalidArgument, "workspace already swapped");
+    if (n >= 0x7FFFFFFFull)
+        fail(kErrInvalidArgument, "num_points must be below 2^31 per call");
+    const size_t need = external_workspace_bytes(n);
+    if (need > bytes)
+        fail(kErrOutOfRange, "external workspace too small");
+    const size_t cap = std::max<size_t>(n, 1024);
+    Workspace w{};
+    w.cap = cap;
+    w.items_cap = cap + cap / kJobPoints + 2;
+    check_hip(rocprim::radix_sort_pairs(nullptr, w.sort_tmp_bytes,
+            (unsigned long long*)nullptr, (unsigned long long*)nullptr,
+            (unsigned*)nullptr, (unsigned*)nullptr, cap, 0u,
+            (unsigned)key_bits_ + 1u, stream_), kErrComputeDevices,
+            "radix sort size query");
+    carve_workspace(static_cast<char*>(base), cap, prec_ == kPrecDouble ?
+            sizeof(RecD) : sizeof(RecF), prec_ == kPrecDouble ?
+            sizeof(double) : sizeof(float), w.sort_tmp_bytes, w.items_cap,
+            w.keys, w.idx, &w.rec, &w.meta, &w.vq, &w.gw, &w.sort_tmp,
+            &w.head, &w.bpos, &w.bstart, &w.scan_part, &w.items);
+    saved_ = save_workspace();
+    set_workspace(w);
+    external_ = true;
+}
+
+void Plan::restore_workspace()
+{
+    if (!external_) return;
+    set_workspace(saved_);
+    external_ = false;
+}
+
 void Plan::free_workspace()
 {
     for (int i = 0; i < 2; ++i)

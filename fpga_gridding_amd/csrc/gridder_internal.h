@@ -279,8 +279,17 @@ public:
     // FP64 index math on FP32 data (created as kPrecSingleIndex64).
     bool index64() const { return index64_; }
     int grid_blocks() const { return grid_blocks_; }
+    int device() const { return device_; }
     // Device bytes of the plan's constant tables (kernel set).
     size_t table_bytes() const { return table_bytes_; }
+    // Per-call workspace carved from device memory the caller provides
+    // (no allocation): bytes a call of n points needs, and swapping such a
+    // workspace in for one call (restore_workspace() puts the plan's own
+    // back; the kernels already enqueued keep the carved one).  The
+    // carved memory must not be in use by another call while this one runs.
+    size_t external_workspace_bytes(size_t n) const;
+    void use_external_workspace(void* base, size_t bytes, size_t n);
+    void restore_workspace();
 
 private:
     void ensure_capacity(size_t n);
@@ -332,7 +341,26 @@ private:
     PlaneInfo* d_pinfo_ = nullptr;
     double* d_normtab_ = nullptr;
 
-    // Workspace (grown on demand).
+    // Workspace (grown on demand), or carved from external memory while
+    // external_ is set (the plan's own pointers then wait in saved_).
+    struct Workspace
+    {
+        size_t cap, items_cap;
+        unsigned long long* keys[2];
+        unsigned* idx[2];
+        void* rec;
+        int4* meta;
+        void* vq;
+        unsigned* gw;
+        void* sort_tmp;
+        size_t sort_tmp_bytes;
+        unsigned *head, *bpos, *bstart, *scan_part;
+        int4* items;
+    };
+    Workspace save_workspace() const;
+    void set_workspace(const Workspace& w);
+    Workspace saved_{};
+    bool external_ = false;
     size_t cap_ = 0;
     size_t items_cap_ = 0;
     unsigned long long* d_keys_[2] = {nullptr, nullptr};

@@ -163,6 +163,23 @@ void gw_grid_device(gw_plan* plan, size_t num_points, const void* d_uu,
         const void* d_vv, const void* d_ww, const void* d_vis, void* d_grid,
         int* status);
 
+/* gw_grid_device without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as gw_grid_device, but the per-call
+ * workspace lives in a static 64 MiB device array of the library (one per
+ * GPU, held until the process ends) instead of the plan's own buffers; the
+ * call allocates and frees nothing.  A call of more points than that array
+ * holds (about 600k points for FP32 plans; the message gives the plan's
+ * limit) fails with OSKAR_ERR_OUT_OF_RANGE.  All calls on one GPU share the
+ * array: calls that overlap on one GPU (on different streams or from
+ * different threads) race and must be serialised by the caller.  The plan
+ * itself is created beforehand (gw_plan_create allocates). */
+GW_EXPORT
+void gw_grid_device_noalloc(gw_plan* plan, size_t num_points,
+        const void* d_uu, const void* d_vv, const void* d_ww,
+        const void* d_vis, void* d_grid, int* status);
+
 /* Wait for the plan's last gw_grid_device call and return its skip count,
  * the increment to add to the caller's norm, and the number of grid
  * updates (sum over gridded points of (2*support+1)^2). */
