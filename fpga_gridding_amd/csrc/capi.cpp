@@ -17,11 +17,14 @@
 #include "oskar_grid_weights.h"
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -61,6 +64,138 @@ struct DevBuf
     }
 };
 
+// Page-locked host buffer that only grows.
+struct PinBuf
+{
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void* get(size_t need)
+    {
+        if (need > bytes)
+        {
+            if (ptr) (void)hipHostFree(ptr);
+            ptr = nullptr;
+            bytes = 0;
+            gw::check_hip(hipHostMalloc(&ptr, need, hipHostMallocDefault),
+                    gw::kErrMemAlloc, "hipHostMalloc");
+            bytes = need;
+        }
+        return ptr;
+    }
+    void release()
+    {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+// A fixed team of host threads for the drop-in's copies between pageable
+// and page-locked memory and its grid additions: run(f) calls f(t, nt) for
+// t = 0 .. nt-1 (t = 0 on the calling thread) and returns when all are
+// done.  Process-wide, created on first use; its threads are detached and
+// never stopped (they idle on a condition variable).  Calls from several
+// host threads take turns.
+class HostPool
+{
+public:
+    static HostPool& get()
+    {
+        static HostPool* pool = new HostPool();
+        return *pool;
+    }
+    int size() const { return nt_; }
+    void run(const std::function<void(int, int)>& f)
+    {
+        std::lock_guard<std::mutex> turn(turn_);
+        if (nt_ == 1)
+        {
+            f(0, 1);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            job_ = &f;
+            pending_ = nt_ - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        f(0, nt_);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    HostPool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        nt_ = (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
+        for (int t = 1; t < nt_; ++t)
+            std::thread([this, t] { work(t); }).detach();
+    }
+    void work(int t)
+    {
+        unsigned seen = 0;
+        for (;;)
+        {
+            const std::function<void(int, int)>* f;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                f = job_;
+            }
+            (*f)(t, nt_);
+            std::lock_guard<std::mutex> lk(m_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    int nt_ = 1;
+    std::mutex turn_, m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int, int)>* job_ = nullptr;
+    unsigned gen_ = 0;
+    int pending_ = 0;
+};
+
+// memcpy of n bytes split over the pool's threads.
+void pool_memcpy(void* dst, const void* src, size_t n)
+{
+    HostPool& pool = HostPool::get();
+    if (n < ((size_t)1 << 20) || pool.size() == 1)
+    {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    pool.run([&](int t, int nt) {
+        const size_t per = (n / nt + 63) & ~(size_t)63;
+        const size_t a = std::min(n, per * t), b = std::min(n, a + per);
+        if (b > a)
+            std::memcpy(static_cast<char*>(dst) + a,
+                        static_cast<const char*>(src) + a, b - a);
+    });
+}
+
+// memcmp == 0 of n bytes split over the pool's threads.
+bool pool_equal(const void* x, const void* y, size_t n)
+{
+    HostPool& pool = HostPool::get();
+    if (n < ((size_t)1 << 20) || pool.size() == 1)
+        return std::memcmp(x, y, n) == 0;
+    std::vector<char> diff(pool.size(), 0);
+    pool.run([&](int t, int nt) {
+        const size_t per = (n / nt + 63) & ~(size_t)63;
+        const size_t a = std::min(n, per * t), b = std::min(n, a + per);
+        if (b > a && std::memcmp(static_cast<const char*>(x) + a,
+                                 static_cast<const char*>(y) + a, b - a))
+            diff[t] = 1;
+    });
+    for (char d : diff)
+        if (d) return false;
+    return true;
+}
+
 // Per-thread state of the drop-in calls.  Deliberately never destroyed at
 // thread or process exit (the HIP runtime may be gone by then); released
 // explicitly by gw_release_cached().
@@ -75,6 +210,29 @@ struct DropinCache
     std::vector<char> cube;
     int device = -1;
     DevBuf in, grid, d_support_hist, d_plane_hist, d_cell_count, d_cell_wsum;
+    // Page-locked staging of the drop-in copies (two slots taking turns)
+    // and the events that free a slot.
+    PinBuf stage[2];
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_live[2] = {false, false};
+    // Waits until slot k is free (its last DMA done).
+    void stage_wait(int k)
+    {
+        if (stage_live[k])
+            gw::check_hip(hipEventSynchronize(stage_ev[k]),
+                    gw::kErrKernelLaunch, "event synchronize");
+    }
+    // Marks slot k busy until the work queued on s so far is done.
+    void stage_mark(int k, hipStream_t s)
+    {
+        if (!stage_ev[k])
+            gw::check_hip(hipEventCreateWithFlags(&stage_ev[k],
+                    hipEventDisableTiming), gw::kErrComputeDevices,
+                    "hipEventCreate");
+        gw::check_hip(hipEventRecord(stage_ev[k], s), gw::kErrKernelLaunch,
+                "event record");
+        stage_live[k] = true;
+    }
     // Uniform weighting.
     std::unique_ptr<gw::Weights> weights;
     int weights_device = -1;
@@ -99,7 +257,7 @@ struct DropinCache
                 G == G_ && cell == cell_ && ws == ws_ && device == dev &&
                 cube.size() == cube_bytes &&
                 std::equal(support.begin(), support.end(), support_) &&
-                std::memcmp(cube.data(), conv, cube_bytes) == 0;
+                pool_equal(cube.data(), conv, cube_bytes);
     }
 };
 
@@ -208,6 +366,92 @@ void weights_read_host(int prec, size_t n, const T* uu, const T* vv,
     *num_skipped = sk;
 }
 
+// Staging slot size of the drop-in copies.
+constexpr size_t kStageBytes = (size_t)32 << 20;
+
+// Host arrays to device through the two page-locked slots: each chunk is
+// copied into a slot by the host pool while the previous chunk's DMA runs.
+struct Upload
+{
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+
+void upload_staged(DropinCache& c, hipStream_t s,
+                   std::initializer_list<Upload> parts)
+{
+    int k = 0;
+    for (const Upload& u : parts)
+    {
+        for (size_t off = 0; off < u.bytes; off += kStageBytes, ++k)
+        {
+            const size_t len = std::min(kStageBytes, u.bytes - off);
+            const int slot = k & 1;
+            c.stage_wait(slot);
+            void* pin = c.stage[slot].get(kStageBytes);
+            pool_memcpy(pin, static_cast<const char*>(u.src) + off, len);
+            gw::check_hip(hipMemcpyAsync(static_cast<char*>(u.dst) + off, pin,
+                    len, hipMemcpyHostToDevice, s), gw::kErrMemCopy,
+                    "upload points");
+            c.stage_mark(slot, s);
+        }
+    }
+}
+
+// The device grid rows (a delta gridded from zero, pitch G cells) added
+// into the caller's grid: only columns col0 .. col0 + cols - 1 come back,
+// in chunks of rows through the two slots, and the host pool adds each
+// chunk's nonzero cells while the next chunk's DMA runs (cells nothing
+// reached stay as the caller left them).
+template <typename T>
+void download_add(DropinCache& c, hipStream_t s, T* grid, const T* d_grid,
+                  int row0, int rows, int col0, int cols, int G)
+{
+    const size_t rb = (size_t)cols * 2 * sizeof(T);
+    const int per = (int)std::max<size_t>(1, kStageBytes / rb);
+    const int chunks = (rows + per - 1) / per;
+    auto issue = [&](int k) {
+        const int slot = k & 1;
+        c.stage_wait(slot);
+        void* pin = c.stage[slot].get(kStageBytes);
+        const int r = k * per;
+        const int nr = std::min(per, rows - r);
+        gw::check_hip(hipMemcpy2DAsync(pin, rb,
+                d_grid + ((size_t)r * G + col0) * 2, (size_t)G * 2 * sizeof(T),
+                rb, (size_t)nr, hipMemcpyDeviceToHost, s), gw::kErrMemCopy,
+                "download grid rows");
+        c.stage_mark(slot, s);
+    };
+    issue(0);
+    if (chunks > 1) issue(1);
+    for (int k = 0; k < chunks; ++k)
+    {
+        const int slot = k & 1;
+        c.stage_wait(slot);
+        const T* pin = static_cast<const T*>(c.stage[slot].ptr);
+        const int r0 = k * per;
+        const int nr = std::min(per, rows - r0);
+        HostPool::get().run([&](int t, int nt) {
+            for (int r = t; r < nr; r += nt)
+            {
+                T* g = grid + ((size_t)(row0 + r0 + r) * G + col0) * 2;
+                const T* d = pin + (size_t)r * cols * 2;
+                for (int i = 0; i < cols; ++i)
+                {
+                    const T re = d[2 * i], im = d[2 * i + 1];
+                    if (re != (T)0 || im != (T)0)
+                    {
+                        g[2 * i] += re;
+                        g[2 * i + 1] += im;
+                    }
+                }
+            }
+        });
+        if (k + 2 < chunks) issue(k + 2);
+    }
+}
+
 // Caller-side diagnostics arrays of oskar_grid_wproj_f_diagnostics.
 struct DiagHost
 {
@@ -265,16 +509,13 @@ void run_dropin(int prec, size_t P, const int* support, int ov, int C,
         return;
     }
 
-    // Inputs: uu | vv | ww | vis, one device allocation.
+    // Inputs: uu | vv | ww | vis, one device allocation, staged through
+    // page-locked memory.
     T* d_in = (T*)c.in.get(5 * n * sizeof(T));
-    gw::check_hip(hipMemcpyAsync(d_in, uu, n * sizeof(T),
-            hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload uu");
-    gw::check_hip(hipMemcpyAsync(d_in + n, vv, n * sizeof(T),
-            hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload vv");
-    gw::check_hip(hipMemcpyAsync(d_in + 2 * n, ww, n * sizeof(T),
-            hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload ww");
-    gw::check_hip(hipMemcpyAsync(d_in + 3 * n, vis, 2 * n * sizeof(T),
-            hipMemcpyHostToDevice, s), gw::kErrMemCopy, "upload vis");
+    upload_staged(c, s, {{d_in, uu, n * sizeof(T)},
+                         {d_in + n, vv, n * sizeof(T)},
+                         {d_in + 2 * n, ww, n * sizeof(T)},
+                         {d_in + 3 * n, vis, 2 * n * sizeof(T)}});
 
     gw::DiagBuffers diag;
     if (dh)
@@ -304,9 +545,17 @@ void run_dropin(int prec, size_t P, const int* support, int ov, int C,
         rows = sm.v_max - sm.v_min + 1;
         const size_t row_vals = 2 * (size_t)G;
         T* d_grid = (T*)c.grid.get((size_t)rows * row_vals * sizeof(T));
-        gw::check_hip(hipMemcpyAsync(d_grid, grid + (size_t)row0 * row_vals,
-                (size_t)rows * row_vals * sizeof(T), hipMemcpyHostToDevice,
-                s), gw::kErrMemCopy, "upload grid rows");
+        // Diagnostics: the caller's rows go up and come back (the
+        // reference's accumulation on the device).  Otherwise the rows are
+        // gridded from zero and the delta added on the host.
+        if (dh)
+            gw::check_hip(hipMemcpyAsync(d_grid, grid + (size_t)row0 *
+                    row_vals, (size_t)rows * row_vals * sizeof(T),
+                    hipMemcpyHostToDevice, s), gw::kErrMemCopy,
+                    "upload grid rows");
+        else
+            gw::check_hip(hipMemsetAsync(d_grid, 0, (size_t)rows * row_vals *
+                    sizeof(T), s), gw::kErrMemCopy, "clear grid rows");
         if (dh)
         {
             const size_t cells = (size_t)rows * G;
@@ -321,9 +570,14 @@ void run_dropin(int prec, size_t P, const int* support, int ov, int C,
         sm = plan.read_summary();
         if (sm.flags & 2u)
             gw::fail(gw::kErrOutOfRange, "internal: work item table overflow");
-        gw::check_hip(hipMemcpyAsync(grid + (size_t)row0 * row_vals, d_grid,
-                (size_t)rows * row_vals * sizeof(T), hipMemcpyDeviceToHost,
-                s), gw::kErrMemCopy, "download grid rows");
+        if (dh)
+            gw::check_hip(hipMemcpyAsync(grid + (size_t)row0 * row_vals,
+                    d_grid, (size_t)rows * row_vals * sizeof(T),
+                    hipMemcpyDeviceToHost, s), gw::kErrMemCopy,
+                    "download grid rows");
+        else
+            download_add(c, s, grid, d_grid, row0, rows, sm.u_min,
+                         sm.u_max - sm.u_min + 1, G);
     }
     if (dh)
     {
@@ -564,6 +818,16 @@ GW_EXPORT void gw_release_cached(void)
         t_cache->d_plane_hist.release();
         t_cache->d_cell_count.release();
         t_cache->d_cell_wsum.release();
+        for (int k = 0; k < 2; ++k)
+        {
+            if (t_cache->stage_ev[k])
+            {
+                (void)hipEventSynchronize(t_cache->stage_ev[k]);
+                (void)hipEventDestroy(t_cache->stage_ev[k]);
+            }
+            t_cache->stage_live[k] = false;
+            t_cache->stage[k].release();
+        }
         delete t_cache;
         t_cache = nullptr;
     }
