@@ -9,12 +9,19 @@ kernel cubes, FITS grid cubes.
   short buffers give the reference's status codes.
 * scale_coords (read_vis.c:49-73): bit-exact against the formula (FP64
   scale, rounded to float for float coordinates).
-* FITS: kernel cubes built here byte by byte from the FITS standard (cards,
-  big-endian data, BINTABLE extension) are read as read_kernel reads them
-  (read_kernel.c:35-118); cubes written by gw_write_fits_cube
-  (write_fits_cube.c) are parsed back with an independent reader, plane by
-  plane.  No reference FITS build exists here (it needs cfitsio), so these
-  are pinned by the standard, not by reference files.
+* FITS against the reference: kernel files written through the reference's
+  vendored cfitsio 3.41 are read exactly as its read_kernel (read_kernel.c:
+  36-118) read them -- FP32 and FP64 cubes bit for bit (including a scaled
+  16-bit image), header values, and a float support column truncated as
+  cfitsio converts to int; cubes written by gw_write_fits_cube carry the
+  same data bytes, sizes and keyword values as the files the reference's
+  write_fits_cube (write_fits_cube.c:37-168) wrote from the same arrays,
+  including a plane rewritten in place (tests/golden/make_fits_golden.py,
+  a gcc build of those sources and cfitsio in oracle/_ref).
+* FITS from the standard: kernel cubes built here byte by byte (cards,
+  big-endian data, BINTABLE extension) cover the other BITPIX values and
+  the error paths; written cubes are also parsed back with an independent
+  reader, plane by plane.
 """
 import os
 import shutil
@@ -116,6 +123,56 @@ def test_scale_coords_matches_formula():
         s = dt(scale)
         assert _bits(u, u0 * s) and _bits(v, (u0 * 2).astype(dt) * s)
         assert _bits(w, (u0 * -0.5).astype(dt) * s)
+
+
+# ---------------------------------------------------------------------------
+# FITS, against files the reference's own FITS code wrote
+# ---------------------------------------------------------------------------
+
+def _header(raw):
+    """Keyword -> value of the first header (comments dropped)."""
+    out = {}
+    for i in range(0, len(raw), 80):
+        c = raw[i:i + 80].decode("ascii")
+        if c.startswith("END"):
+            return out, ((i + 80 + 2879) // 2880) * 2880
+        if c[8:10] == "= ":
+            out[c[:8].strip()] = c[10:].split("/")[0].strip()
+    raise AssertionError("no END card")
+
+
+@pytest.mark.parametrize("tag", ["e", "j"])
+@pytest.mark.parametrize("pn,prec", [("f", gio.OSKAR_SINGLE),
+                                     ("d", gio.OSKAR_DOUBLE)])
+def test_read_kernel_matches_reference_read_kernel(tag, pn, prec):
+    d = np.load(os.path.join(GOLD, "ref_fits.npz"), allow_pickle=False)
+    cube, hdr, sup = gio.read_kernel(os.path.join(
+        GOLD, f"ref_kernel_{tag}.fits"), prec)
+    assert _bits(cube, d[f"kernel_{tag}_cube_{pn}"])
+    assert _bits(sup, d[f"kernel_{tag}_support_{pn}"])
+    C, P, ov, G = (int(v) for v in d[f"kernel_{tag}_header_{pn}"])
+    cell, ws = (float(v) for v in d[f"kernel_{tag}_cell_ws_{pn}"])
+    assert hdr == dict(conv_size_half=C, num_w_planes=P, oversample=ov,
+                       grid_size=G, cellsize_rad=cell, w_scale=ws)
+
+
+def test_write_fits_cube_matches_reference_files(tmp_path):
+    d = np.load(os.path.join(GOLD, "ref_fits.npz"), allow_pickle=False)
+    root = str(tmp_path / "g")
+    gio.write_fits_cube(d["grid_c"], root, 7, 5, 2, -1, complex_=True)
+    pairs = [(root + s, os.path.join(GOLD, "ref_grid" + s))
+             for s in ("_REAL.fits", "_IMAG.fits")]
+    path = str(tmp_path / "c.fits")
+    gio.write_fits_cube(d["cube_d"], path, 7, 5, 3, -1)
+    gio.write_fits_cube(d["plane_d"], path, 7, 5, 3, 1)
+    pairs.append((path, os.path.join(GOLD, "ref_cube_d.fits")))
+    for ours, ref in pairs:
+        a, b = open(ours, "rb").read(), open(ref, "rb").read()
+        assert len(a) == len(b), ref
+        ha, da = _header(a)
+        hb, db = _header(b)
+        assert ha == hb, ref
+        assert a[da:] == b[db:], ref
 
 
 # ---------------------------------------------------------------------------
