@@ -553,6 +553,27 @@ __device__ __forceinline__ unsigned table_ref(const PlaneInfo& pi, int off_u,
            (off_v < 0 ? 1u << 31 : 0u);
 }
 
+// Control bits of a sorted point, in meta.z above the plane (bits 0-15)
+// and the support rank (bits 16-29): the point's footprint origin differs
+// from the previous sorted point's (the scatter adds the previous point's
+// sums into the region before this point), and its tap table differs (the
+// scatter loads other taps).  Within one stream's share of a segment the
+// previous sorted point is the previous point the stream took.
+constexpr unsigned kNewPos = 1u << 30;
+constexpr unsigned kNewTab = 1u << 31;
+constexpr unsigned kRankMask = 0x3FFFu;
+
+// meta.x of a point for the LDS scatter: its origin's byte offset in the
+// region of its tile (tiles start at multiples of the tile side T, region
+// pitch R cells, FP64 cells); R == 0: the origin u | v << 16 itself.
+__device__ __forceinline__ int region_offset(int pos, int R, int tmask)
+{
+    if (R <= 0) return pos;
+    const int u = pos & 0xFFFF;
+    const int v = (int)((unsigned)pos >> 16);
+    return ((v & tmask) * R + (u & tmask)) * (int)sizeof(double);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather(
         const Summary* __restrict__ summary,
@@ -561,7 +582,8 @@ __global__ void __launch_bounds__(256) k_gather(
         const typename Types<T>::Rec* __restrict__ rec,
         const unsigned* __restrict__ gw, const PlaneInfo* __restrict__ pinfo,
         int h, int K, int tile_shift, int4* __restrict__ meta,
-        typename Types<T>::T4* __restrict__ vq, unsigned* __restrict__ head)
+        typename Types<T>::T4* __restrict__ vq, unsigned* __restrict__ head,
+        int R, int tmask)
 {
     const unsigned nv = summary->nv32;
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
@@ -577,8 +599,19 @@ __global__ void __launch_bounds__(256) k_gather(
         const int off_u = (r.sub & 0xFF) - 128;
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
         const PlaneInfo& pi = pinfo[plane];
-        meta[s] = make_int4(r.pos, (int)table_ref(pi, off_u, off_v, h),
-                plane | (pi.srank << 16),
+        const unsigned tref = table_ref(pi, off_u, off_v, h);
+        unsigned flags = kNewPos | kNewTab;
+        if (s > 0)
+        {
+            const typename Types<T>::Rec q = rec[idx[s - 1]];
+            const int qp = (unsigned)q.sub >> 17;
+            if (q.pos == r.pos) flags &= ~kNewPos;
+            if (table_ref(pinfo[qp], (q.sub & 0xFF) - 128,
+                          ((q.sub >> 8) & 0xFF) - 128, h) == tref)
+                flags &= ~kNewTab;
+        }
+        meta[s] = make_int4(region_offset(r.pos, R, tmask), (int)tref,
+                (int)((unsigned)plane | ((unsigned)pi.srank << 16) | flags),
                 gw ? (int)min(gw[i], 0x7FFFFFFFu) : 0);
         typename Types<T>::T4 q;
         q.x = r.re;
@@ -888,12 +921,12 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
 #pragma unroll
         for (int s = 0; s < NS; ++s) loff[s] *= (unsigned)sizeof(double);
     }
+    // LDS: meta.x is already the origin's byte offset in the region (the
+    // gather stage's region_offset); otherwise the origin u | v << 16.
     auto origin = [&](int pos) -> unsigned {
         const int u0 = pos & 0xFFFF;
         const int v0 = (int)((unsigned)pos >> 16);
-        return LDS ? (unsigned)((v0 - x.vy0) * x.R + (u0 - x.ux0)) *
-                        (unsigned)sizeof(double) :
-                (unsigned)((v0 - x.row0) * x.G + u0);
+        return LDS ? (unsigned)pos : (unsigned)((v0 - x.row0) * x.G + u0);
     };
     unsigned cnt = 0;
     unsigned long long wsum = 0;
@@ -993,6 +1026,164 @@ __device__ __forceinline__ void scatter_stream(const ScatterCtx<T>& x,
     flush(base);
 }
 
+// The LDS-region stream with the per-point control precomputed by the
+// gather stage (meta.x the origin's byte offset in the region, meta.z's
+// kNewPos / kNewTab bits): per point one record read (two scalar loads at
+// one shared offset), three bit tests and the loop test on the scalar
+// unit; the lane work as in scatter_stream (same lane map, taps, sums,
+// flushes and results).  The loop is unrolled three times so that the
+// records of the current, next and following points rotate through three
+// register sets without moves.
+template <int NS, typename TapT>
+__device__ __forceinline__ void scatter_stream_lds(const ScatterCtx<float>& x,
+        const PlaneInfo& pi, int wis, int a, int b, int cb)
+{
+    const int lane = threadIdx.x & 63;
+    const int W = pi.W, W2 = W * W;
+    const int stride = 64 * pi.wpp;
+    const char* __restrict__ ptab = static_cast<const char*>(x.etab);
+    unsigned toff[NS], k2[NS], loff[NS];
+    bool valid[NS];
+    const int c0 = cb + wis * 64 + lane;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+    {
+        const int c = c0 + s * stride;
+        valid[s] = c < W2;
+        const unsigned cc = valid[s] ? (unsigned)c : 0u;
+        const unsigned j = pi.wmagic ? __umulhi(cc, pi.wmagic) :
+                cc / (unsigned)W;
+        const unsigned k = cc - j * (unsigned)W;
+        loff[s] = (j * (unsigned)x.R + k) * (unsigned)sizeof(double);
+        toff[s] = cc * (unsigned)sizeof(TapT);
+        k2[s] = 2 * k * (unsigned)sizeof(TapT);
+    }
+    const unsigned bu = (unsigned)(W - 1) * (unsigned)sizeof(TapT);
+    const unsigned bv = (unsigned)((W - 1) * W) * (unsigned)sizeof(TapT);
+    const unsigned bb = (unsigned)(W2 - 1) * (unsigned)sizeof(TapT);
+    float tre[NS], tim[NS], nre[NS], nim[NS], acc_re[NS], acc_im[NS];
+    auto load_taps = [&](unsigned table) {
+        const unsigned e = table & 0x3FFFFFFFu;
+        const unsigned flip = table >> 30;
+        const char* tp = ptab + (size_t)e * sizeof(TapT);
+        auto issue = [&](int s, unsigned o) {
+            const auto t = ld_tap(static_cast<const TapT*>(nullptr), tp + o);
+            nre[s] = t.x;
+            nim[s] = t.y;
+        };
+        if (flip == 0)
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) issue(s, toff[s]);
+        }
+        else if (flip == 1)
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) issue(s, toff[s] + bu - k2[s]);
+        }
+        else if (flip == 2)
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) issue(s, bv - toff[s] + k2[s]);
+        }
+        else
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) issue(s, bb - toff[s]);
+        }
+    };
+    auto take_taps = [&]() {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            tre[s] = nre[s];
+            tim[s] = nim[s];
+        }
+    };
+    const unsigned sink8 = (unsigned)(x.R * x.R + lane) *
+            (unsigned)sizeof(double);
+    auto flush = [&](unsigned base) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            lds_add_cell(x.lds, valid[s] ? base + loff[s] : sink8,
+                         (double)acc_re[s], (double)acc_im[s]);
+    };
+    // Records through the constant address space at a 32-bit byte offset
+    // shared by the two arrays (16 bytes per point each).
+    const ConstPtr<char> mbase = as_const<char>(x.meta);
+    const ConstPtr<char> vbase = as_const<char>(x.vq);
+    struct Rec
+    {
+        unsigned base, tref, z;
+        float vx, vy, vz, vw;
+    };
+    auto load = [&](int i) {
+        const unsigned o = (unsigned)i * 16u;
+        const ConstPtr<unsigned> m =
+                reinterpret_cast<ConstPtr<unsigned>>(mbase + o);
+        const ConstPtr<float> v = reinterpret_cast<ConstPtr<float>>(vbase + o);
+        Rec r;
+        r.base = m[0];
+        r.tref = m[1];
+        r.z = m[2];
+        r.vx = v[0];
+        r.vy = v[1];
+        r.vz = v[2];
+        r.vw = v[3];
+        return r;
+    };
+    Rec A = load(a), B = A, C = A;
+    if (a + 1 < b) B = load(a + 1);
+    load_taps(A.tref);
+    take_taps();
+    bool fresh = true;
+    int k = a;
+    // Point k from cur; nxt holds point k + 1, fut receives point k + 2.
+    // Returns false after the last point.
+    auto step = [&](const Rec& cur, const Rec& nxt, Rec& fut) -> bool {
+        const bool more = k + 1 < b;
+        if (k + 2 < b) fut = load(k + 2);
+        const bool nt = more && (nxt.z & kNewTab);
+        if (nt) load_taps(nxt.tref);
+        if (fresh)
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+            {
+                acc_re[s] = fmaf(cur.vy, tim[s], cur.vx * tre[s]);
+                acc_im[s] = fmaf(cur.vw, tim[s], cur.vz * tre[s]);
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+            {
+                acc_re[s] = fmaf(cur.vx, tre[s], acc_re[s]);
+                acc_re[s] = fmaf(cur.vy, tim[s], acc_re[s]);
+                acc_im[s] = fmaf(cur.vz, tre[s], acc_im[s]);
+                acc_im[s] = fmaf(cur.vw, tim[s], acc_im[s]);
+            }
+        }
+        if (!more)
+        {
+            flush(cur.base);
+            return false;
+        }
+        fresh = (nxt.z & kNewPos) != 0;
+        if (fresh) flush(cur.base);
+        if (nt) take_taps();
+        ++k;
+        return true;
+    };
+    for (;;)
+    {
+        if (!step(A, B, C)) break;
+        if (!step(B, C, A)) break;
+        if (!step(C, A, B)) break;
+    }
+}
+
 template <typename T> constexpr int slots_upto(int n)
 {
     return n < slots_max<T>() ? n : slots_max<T>();
@@ -1004,8 +1195,10 @@ __device__ __forceinline__ void scatter_dispatch(const ScatterCtx<T>& x,
 {
     // The plan never sets nslot above slots_max<T>() (<= 8).
     static_assert(slots_max<T>() <= 8, "dispatch covers 1..8 slots");
-#define GW_STREAM(n) scatter_stream<T, DIAG, LDS, slots_upto<T>(n), TapT, \
-        COMP>(x, pi, wis, a, b, cb)
+#define GW_STREAM(n) do { if constexpr (LDS && !DIAG && sizeof(T) == 4) \
+        scatter_stream_lds<slots_upto<T>(n), TapT>(x, pi, wis, a, b, cb); \
+        else scatter_stream<T, DIAG, LDS, slots_upto<T>(n), TapT, COMP>( \
+        x, pi, wis, a, b, cb); } while (0)
     switch (pi.nslot)
     {
     case 1: GW_STREAM(1); break;
@@ -1112,7 +1305,8 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
             const int z = meta[start + i].z;
             myplane[q] = z & 0xFFFF;
             myhead[q] = (i == 0) ||
-                    (meta[start + i - 1].z >> 16) != (z >> 16);
+                    ((meta[start + i - 1].z >> 16) & kRankMask) !=
+                    ((z >> 16) & kRankMask);
             heads += myhead[q] ? 1u : 0u;
         }
     }
@@ -2167,7 +2361,7 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
             (const unsigned long long*)d_keys_[1], (const unsigned*)d_idx_[1],
             (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
             (const PlaneInfo*)d_pinfo_, h_, K_, tile_shift_, d_meta_,
-            (T4*)d_vq_, d_head_);
+            (T4*)d_vq_, d_head_, diag ? 0 : region_, (1 << tile_bits_) - 1);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_gather launch");
     // Tiles: exclusive scan of the heads, then their starts.
     scan_u32(d_head_, d_bpos_, &d_summary_->nv32, n, &d_summary_->n_buckets);
@@ -2533,16 +2727,36 @@ __global__ void __launch_bounds__(256) k_gather_batch(
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
          s += gridDim.x * blockDim.x)
     {
-        const BatchJob& d = jobs[batch_job_of_tile(jobs, njobs,
-                (unsigned)(keys[s] >> lmax))];
+        const int jd = batch_job_of_tile(jobs, njobs,
+                (unsigned)(keys[s] >> lmax));
+        const BatchJob& d = jobs[jd];
         const RecF r = rec[idx[s]];
         const int plane = (unsigned)r.sub >> 17;
         const float sgn = ((r.sub >> 16) & 1) ? -1.0f : 1.0f;
         const int off_u = (r.sub & 0xFF) - 128;
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
         const PlaneInfo& pi = d.pinfo[plane];
-        meta[s] = make_int4(r.pos, (int)table_ref(pi, off_u, off_v, d.sub_h),
-                plane | (pi.srank << 16), 0);
+        const unsigned tref = table_ref(pi, off_u, off_v, d.sub_h);
+        unsigned flags = kNewPos | kNewTab;
+        if (s > 0 && batch_job_of_tile(jobs, njobs,
+                (unsigned)(keys[s - 1] >> lmax)) == jd)
+        {
+            // The previous sorted point, when it is of the same job (its
+            // plane indexes this job's tables; across jobs the work items
+            // split anyway).
+            const RecF q = rec[idx[s - 1]];
+            const int qp = (unsigned)q.sub >> 17;
+            {
+                if (q.pos == r.pos) flags &= ~kNewPos;
+                if (table_ref(d.pinfo[qp], (q.sub & 0xFF) - 128,
+                              ((q.sub >> 8) & 0xFF) - 128, d.sub_h) == tref)
+                    flags &= ~kNewTab;
+            }
+        }
+        meta[s] = make_int4(region_offset(r.pos, d.R,
+                (1 << d.tile_bits) - 1), (int)tref,
+                (int)((unsigned)plane | ((unsigned)pi.srank << 16) | flags),
+                0);
         vq[s] = make_float4(r.re, -sgn * r.im, r.im, sgn * r.re);
     }
 }
