@@ -116,6 +116,34 @@ __device__ __forceinline__ float2 ld_tap(const BF16Pair*, const char* p)
                        bf16_to_float((unsigned short)(u >> 16)));
 }
 
+// The same as loaded (raw) and as used (widened): the LDS scatter keeps
+// the next table's taps raw while they are in flight, so that a 16-bit
+// pair's conversion waits for its load only when the taps are taken.
+__device__ __forceinline__ float2 ld_tap_raw(const float2*, const char* p)
+{
+    return ld_tap(static_cast<const float2*>(nullptr), p);
+}
+__device__ __forceinline__ float2 tap_widen(const float2*, float2 t)
+{
+    return t;
+}
+template <typename P16>
+__device__ __forceinline__ unsigned ld_tap_raw(const P16*, const char* p)
+{
+    return *as_global(reinterpret_cast<const unsigned*>(p));
+}
+__device__ __forceinline__ float2 tap_widen(const HalfPair*, unsigned u)
+{
+    HalfPair h;
+    __builtin_memcpy(&h, &u, 4);
+    return tap_pair(h);
+}
+__device__ __forceinline__ float2 tap_widen(const BF16Pair*, unsigned u)
+{
+    return make_float2(bf16_to_float((unsigned short)(u & 0xFFFFu)),
+                       bf16_to_float((unsigned short)(u >> 16)));
+}
+
 // One component of a compensated two-float grid cell, p[0] = hi and
 // p[1] = lo (value hi + lo).  x is split into h = (float)x and
 // l = (float)(x - h); h goes into hi with an atomic add that returns the
@@ -1061,15 +1089,16 @@ __device__ __forceinline__ void scatter_stream_lds(const ScatterCtx<float>& x,
     const unsigned bu = (unsigned)(W - 1) * (unsigned)sizeof(TapT);
     const unsigned bv = (unsigned)((W - 1) * W) * (unsigned)sizeof(TapT);
     const unsigned bb = (unsigned)(W2 - 1) * (unsigned)sizeof(TapT);
-    float tre[NS], tim[NS], nre[NS], nim[NS], acc_re[NS], acc_im[NS];
+    typedef decltype(ld_tap_raw(static_cast<const TapT*>(nullptr),
+                                nullptr)) Raw;
+    float tre[NS], tim[NS], acc_re[NS], acc_im[NS];
+    Raw nraw[NS];           // the next table's taps, in flight
     auto load_taps = [&](unsigned table) {
         const unsigned e = table & 0x3FFFFFFFu;
         const unsigned flip = table >> 30;
         const char* tp = ptab + (size_t)e * sizeof(TapT);
         auto issue = [&](int s, unsigned o) {
-            const auto t = ld_tap(static_cast<const TapT*>(nullptr), tp + o);
-            nre[s] = t.x;
-            nim[s] = t.y;
+            nraw[s] = ld_tap_raw(static_cast<const TapT*>(nullptr), tp + o);
         };
         if (flip == 0)
         {
@@ -1096,8 +1125,10 @@ __device__ __forceinline__ void scatter_stream_lds(const ScatterCtx<float>& x,
 #pragma unroll
         for (int s = 0; s < NS; ++s)
         {
-            tre[s] = nre[s];
-            tim[s] = nim[s];
+            const float2 t = tap_widen(static_cast<const TapT*>(nullptr),
+                                       nraw[s]);
+            tre[s] = t.x;
+            tim[s] = t.y;
         }
     };
     const unsigned sink8 = (unsigned)(x.R * x.R + lane) *
@@ -1404,33 +1435,50 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
 // contiguous ranges from its own queue (s->item_next[x], zero at launch), so
 // its L2 holds the tap tables of that region only, then helps the other
 // queues in turn.  Every workgroup must call this (it synchronises).
+// f is called from one place only, so that it is inlined once: an
+// out-of-line f would receive the LDS region through a generic pointer and
+// its atomics would become flat atomics.
 template <typename F>
 __device__ __forceinline__ void for_each_item(Summary* __restrict__ s,
         int xcds, unsigned* s_next, F&& f)
 {
     const unsigned total = s->total_items;
-    if (xcds <= 1 || xcds > kXcdGroups)
+    const bool queues = xcds > 1 && xcds <= kXcdGroups;
+    const unsigned nq = queues ? (unsigned)xcds : 1u;
+    const unsigned per = (total + nq - 1) / nq;
+    const unsigned home = blockIdx.x % nq;
+    unsigned next = blockIdx.x;     // grid-stride order
+    unsigned q = 0;                 // queues: the q-th after the home queue
+    for (;;)
     {
-        for (unsigned it = blockIdx.x; it < total; it += gridDim.x) f(it);
-        return;
-    }
-    const unsigned per = (total + xcds - 1) / xcds;
-    const unsigned home = blockIdx.x % xcds;
-    for (int q = 0; q < xcds; ++q)
-    {
-        const unsigned xq = (home + q) % xcds;
-        const unsigned lo = xq * per;
-        const unsigned hi = min(total, lo + per);
-        for (;;)
+        unsigned it = ~0u;
+        if (!queues)
         {
-            if (threadIdx.x == 0) *s_next = lo + atomicAdd(&s->item_next[xq], 1u);
-            __syncthreads();
-            const unsigned it = (unsigned)__builtin_amdgcn_readfirstlane(
-                    (int)*s_next);
-            __syncthreads();
-            if (it >= hi) break;
-            f(it);
+            it = next < total ? next : ~0u;
+            next += gridDim.x;
         }
+        else
+        {
+            for (; q < nq; ++q)
+            {
+                const unsigned xq = (home + q) % nq;
+                const unsigned lo = xq * per;
+                const unsigned hi = min(total, lo + per);
+                if (threadIdx.x == 0)
+                    *s_next = lo + atomicAdd(&s->item_next[xq], 1u);
+                __syncthreads();
+                const unsigned got = (unsigned)__builtin_amdgcn_readfirstlane(
+                        (int)*s_next);
+                __syncthreads();
+                if (got < hi)
+                {
+                    it = got;
+                    break;
+                }
+            }
+        }
+        if (it == ~0u) break;
+        f(it);
     }
 }
 
