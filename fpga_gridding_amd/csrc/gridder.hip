@@ -611,7 +611,7 @@ __global__ void __launch_bounds__(256) k_gather(
         const unsigned* __restrict__ gw, const PlaneInfo* __restrict__ pinfo,
         int h, int K, int tile_shift, int4* __restrict__ meta,
         typename Types<T>::T4* __restrict__ vq, unsigned* __restrict__ head,
-        int R, int tmask)
+        int R, int tmask, int pos_shift)
 {
     const unsigned nv = summary->nv32;
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
@@ -628,15 +628,16 @@ __global__ void __launch_bounds__(256) k_gather(
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
         const PlaneInfo& pi = pinfo[plane];
         const unsigned tref = table_ref(pi, off_u, off_v, h);
+        // From the previous sorted key (bits from the top: tile, support
+        // rank, position, plane, sub-cell table): the same tile and
+        // position, the same plane and sub-cell table (hence the same table
+        // reference).
         unsigned flags = kNewPos | kNewTab;
         if (s > 0)
         {
-            const typename Types<T>::Rec q = rec[idx[s - 1]];
-            const int qp = (unsigned)q.sub >> 17;
-            if (q.pos == r.pos) flags &= ~kNewPos;
-            if (table_ref(pinfo[qp], (q.sub & 0xFF) - 128,
-                          ((q.sub >> 8) & 0xFF) - 128, h) == tref)
-                flags &= ~kNewTab;
+            const unsigned long long d = keys[s - 1] ^ key;
+            if ((d >> pos_shift) == 0) flags &= ~kNewPos;
+            if ((d & ((1ull << pos_shift) - 1)) == 0) flags &= ~kNewTab;
         }
         meta[s] = make_int4(region_offset(r.pos, R, tmask), (int)tref,
                 (int)((unsigned)plane | ((unsigned)pi.srank << 16) | flags),
@@ -2409,7 +2410,8 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
             (const unsigned long long*)d_keys_[1], (const unsigned*)d_idx_[1],
             (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
             (const PlaneInfo*)d_pinfo_, h_, K_, tile_shift_, d_meta_,
-            (T4*)d_vq_, d_head_, diag ? 0 : region_, (1 << tile_bits_) - 1);
+            (T4*)d_vq_, d_head_, diag ? 0 : region_, (1 << tile_bits_) - 1,
+            combo_bits_ + plane_bits_);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_gather launch");
     // Tiles: exclusive scan of the heads, then their starts.
     scan_u32(d_head_, d_bpos_, &d_summary_->nv32, n, &d_summary_->n_buckets);
@@ -2775,8 +2777,9 @@ __global__ void __launch_bounds__(256) k_gather_batch(
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
          s += gridDim.x * blockDim.x)
     {
+        const unsigned long long key = keys[s];
         const int jd = batch_job_of_tile(jobs, njobs,
-                (unsigned)(keys[s] >> lmax));
+                (unsigned)(key >> lmax));
         const BatchJob& d = jobs[jd];
         const RecF r = rec[idx[s]];
         const int plane = (unsigned)r.sub >> 17;
@@ -2785,21 +2788,15 @@ __global__ void __launch_bounds__(256) k_gather_batch(
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
         const PlaneInfo& pi = d.pinfo[plane];
         const unsigned tref = table_ref(pi, off_u, off_v, d.sub_h);
+        // From the previous sorted key when it is of the same global tile
+        // (hence job): below lmax the job's own key bits without the tile
+        // (support rank, position, plane, sub-cell table).
         unsigned flags = kNewPos | kNewTab;
-        if (s > 0 && batch_job_of_tile(jobs, njobs,
-                (unsigned)(keys[s - 1] >> lmax)) == jd)
+        if (s > 0)
         {
-            // The previous sorted point, when it is of the same job (its
-            // plane indexes this job's tables; across jobs the work items
-            // split anyway).
-            const RecF q = rec[idx[s - 1]];
-            const int qp = (unsigned)q.sub >> 17;
-            {
-                if (q.pos == r.pos) flags &= ~kNewPos;
-                if (table_ref(d.pinfo[qp], (q.sub & 0xFF) - 128,
-                              ((q.sub >> 8) & 0xFF) - 128, d.sub_h) == tref)
-                    flags &= ~kNewTab;
-            }
+            const unsigned long long dk = keys[s - 1] ^ key;
+            if ((dk >> d.pos_shift) == 0) flags &= ~kNewPos;
+            if ((dk & ((1ull << d.pos_shift) - 1)) == 0) flags &= ~kNewTab;
         }
         meta[s] = make_int4(region_offset(r.pos, d.R,
                 (1 << d.tile_bits) - 1), (int)tref,
@@ -2983,6 +2980,7 @@ void Batch::prepare()
         d.min_share = pl.min_share();
         d.sub_h = pl.half_sub();
         d.sub_k = pl.sub_tables();
+        d.pos_shift = pl.tile_shift() - 2 * d.tile_bits - pl.srank_bits();
         d.tile_base = (unsigned)tiles;
         d.ntiles = 1u << (2 * (pl.coord_bits() - pl.tile_bits()));
         d.point_base = (unsigned)pts;

@@ -395,6 +395,7 @@ struct BatchJob
     unsigned ntiles;          // 4^(coord_bits - tile_bits)
     unsigned point_base;      // first global point of the job
     unsigned n;
+    int pos_shift;            // position bits start here in the job's keys
 };
 
 // Many independent FP32 gridding jobs (each with its own plan: kernel set,

@@ -118,6 +118,61 @@ def test_reduced_precision_full_scale(full_scale, prec, name):
           f"{err_stored:.2e}, on the original inputs {err_orig:.2e}")
 
 
+@pytest.mark.parametrize("prec,name", [(fg.GW_HALF, "fp16"),
+                                       (fg.GW_BFLOAT16, "bf16")],
+                         ids=["fp16", "bf16"])
+def test_compensated_grid_beats_fp32_grid(full_scale, prec, name):
+    """What the compensated two-float grid of a 16-bit plan buys: on the same
+    stored (16-bit) inputs, an FP32 grid from GW_SINGLE_INDEX64 (the same
+    FP64 index math, FP32 products and FP32 position-run sums, FP64 region)
+    and the compensated grid (hi + lo read in float64) against the FP64
+    brute force on those inputs (oracle.cells_d), over 256 covered cells:
+    the compensated grid's RMS error is below the FP32 grid's, because it
+    drops the rounding of every region add into a float cell."""
+    wl, _, _ = full_scale
+    G = wl.grid_size
+    stored = dataclasses.replace(
+        wl, vis=fg.from_storage(fg.to_storage(wl.vis, prec), prec),
+        conv_func=fg.from_storage(fg.to_storage(wl.conv_func, prec), prec))
+    args = (wl.num_w_planes, wl.support, wl.oversample, wl.conv_size_half)
+    cell = float(np.float32(wl.cell_size_rad))
+    wsc = float(np.float32(wl.w_scale))
+    d = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww)]
+    p16 = fg.Plan(prec, *args, wl.conv_func, cell, wsc, G)
+    p32 = fg.Plan(fg.GW_SINGLE_INDEX64, *args,
+                  np.ascontiguousarray(stored.conv_func, np.float32), cell,
+                  wsc, G)
+    d_v16 = fg.DeviceArray.from_host(fg.to_storage(wl.vis, prec))
+    d_v32 = fg.DeviceArray.from_host(
+        np.ascontiguousarray(stored.vis, np.float32))
+    d_comp = fg.DeviceArray(16 * G * G)
+    d_g32 = fg.DeviceArray(8 * G * G)
+    try:
+        d_comp.zero()
+        d_g32.zero()
+        p16.grid(wl.num_points, *d, d_v16, d_comp)
+        r16 = p16.results()
+        p32.grid(wl.num_points, *d, d_v32, d_g32)
+        r32 = p32.results()
+        comp = d_comp.download(np.empty(4 * G * G, np.float32))
+        g32 = d_g32.download(np.empty(2 * G * G, np.float32))
+    finally:
+        for x in (*d, d_v16, d_v32, d_comp, d_g32):
+            x.close()
+        p16.close()
+        p32.close()
+    assert r16[0] == r32[0] and r16[2] == r32[2]
+    cells, gmax = _spot_cells(g32, 256, seed=3)
+    ref = oracle.cells_d(stored, cells)
+    q = comp.reshape(-1, 4)[cells].astype(np.float64)
+    v16 = np.stack([q[:, 0] + q[:, 1], q[:, 2] + q[:, 3]], 1).reshape(-1)
+    e16 = float(np.sqrt(np.mean((v16 - ref) ** 2)) / gmax)
+    e32 = float(np.sqrt(np.mean((_cells_of(g32, cells) - ref) ** 2)) / gmax)
+    print(f"{name} stored inputs, RMS error over 256 cells / max|grid|: "
+          f"compensated {e16:.3e}, FP32 grid {e32:.3e}")
+    assert e16 < e32
+
+
 # --------------------------------------------------------------------------
 # The 64-job mix: batched and served
 # --------------------------------------------------------------------------
