@@ -13,8 +13,10 @@
 //   k_plan     points gridded, update count and norm increment (histogram x
 //              per-(plane, sub-cell) tap sums, fixed summation order);
 //   k_gather   sorted point records (origin, sub-cell table, plane; the
-//              visibility pre-multiplied by the conjugation sign) and the
-//              heads of the tile buckets;
+//              visibility pre-multiplied by the conjugation sign), one per
+//              run of equal keys (same origin, plane and sub-cell table:
+//              the run's visibility terms summed), and the heads of the
+//              tile buckets;
 //   scans      bucket starts and jobs (<= kJobPoints points of one tile);
 //   k_grid     the scatter, one job per workgroup, into an FP64 LDS copy of
 //              the tile's grid region, added to the grid in HBM once per job.
@@ -565,8 +567,8 @@ __global__ void __launch_bounds__(1024) k_plan(
 // (off_v + h) * K + (off_u + h), plane, grid_w); vq = (re, -s*im, im, s*re)
 // with s = -1 when ww > 0 (oskar_grid_wproj.c:172, :233), so that
 //   grid_re += vq.x * c_re + vq.y * c_im,  grid_im += vq.z * c_re + vq.w * c_im
-// with (c_re, c_im) the tap as stored in the kernel cube.  head[s] = 1 where
-// a new tile starts (key bits above tile_shift).
+// with (c_re, c_im) the tap as stored in the kernel cube.  head[m] = 1 where
+// record m starts a new tile (key bits above tile_shift).
 // meta.y of a point: bits 0-29 the element offset of its (plane, |off_v|,
 // |off_u|) tap table (tables are below 4 GiB and taps at least 4 bytes),
 // bit 30 set for off_u < 0, bit 31 for off_v < 0 (the scatter then reads
@@ -602,6 +604,42 @@ __device__ __forceinline__ int region_offset(int pos, int R, int tmask)
     return ((v & tmask) * R + (u & tmask)) * (int)sizeof(double);
 }
 
+// Runs of equal sort keys: mh[s] = 1 where sorted point s starts one (its
+// key differs from the previous point's), or everywhere when merge == 0.
+// Points of one run share the tile, footprint origin, plane and sub-cell
+// table, so they use the same taps at the same cells: the scatter takes
+// one record per run, its visibility terms summed (k_gather).
+__global__ void __launch_bounds__(256) k_merge_heads(
+        const Summary* __restrict__ summary,
+        const unsigned long long* __restrict__ keys,
+        unsigned* __restrict__ mh, int merge)
+{
+    const unsigned nv = summary->nv32;
+    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
+         s += gridDim.x * blockDim.x)
+        mh[s] = (!merge || s == 0 || keys[s - 1] != keys[s]) ? 1u : 0u;
+}
+
+// rstart[m] = first sorted point of run m (mh, run starts; mpos, their
+// exclusive scan); rstart[runs] = the point count.
+__global__ void __launch_bounds__(256) k_run_starts(
+        const Summary* __restrict__ summary, const unsigned* __restrict__ mh,
+        const unsigned* __restrict__ mpos, unsigned* __restrict__ rstart)
+{
+    const unsigned nv = summary->nv32;
+    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
+         s += gridDim.x * blockDim.x)
+        if (mh[s]) rstart[mpos[s]] = s;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        rstart[summary->nm32] = nv;
+}
+
+// One record per run of equal sort keys, run m = sorted points
+// [rstart[m], rstart[m + 1]).  The run's vq terms are summed component by
+// component: each point's (re, -s*im, im, s*re) applies its own
+// conjugation sign, so the sum is exact algebra for any mix of signs
+// (summed in FP64 in sorted order, then stored in T).  mkeys[m] = the
+// run's key.
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather(
         const Summary* __restrict__ summary,
@@ -611,19 +649,22 @@ __global__ void __launch_bounds__(256) k_gather(
         const unsigned* __restrict__ gw, const PlaneInfo* __restrict__ pinfo,
         int h, int K, int tile_shift, int4* __restrict__ meta,
         typename Types<T>::T4* __restrict__ vq, unsigned* __restrict__ head,
-        int R, int tmask, int pos_shift)
+        int R, int tmask, int pos_shift, const unsigned* __restrict__ rstart,
+        unsigned long long* __restrict__ mkeys)
 {
-    const unsigned nv = summary->nv32;
-    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
-         s += gridDim.x * blockDim.x)
+    typedef typename Types<T>::Rec Rec;
+    const unsigned nm = summary->nm32;
+    for (unsigned m = blockIdx.x * blockDim.x + threadIdx.x; m < nm;
+         m += gridDim.x * blockDim.x)
     {
+        const unsigned s = rstart[m], e = rstart[m + 1];
         const unsigned long long key = keys[s];
-        head[s] = (s == 0 || (keys[s - 1] >> tile_shift) !=
+        mkeys[m] = key;
+        head[m] = (s == 0 || (keys[s - 1] >> tile_shift) !=
                 (key >> tile_shift)) ? 1u : 0u;
         const unsigned i = idx[s];
-        const typename Types<T>::Rec r = rec[i];
+        const Rec r = rec[i];
         const int plane = (unsigned)r.sub >> 17;
-        const T sgn = ((r.sub >> 16) & 1) ? (T)-1 : (T)1;
         const int off_u = (r.sub & 0xFF) - 128;
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
         const PlaneInfo& pi = pinfo[plane];
@@ -631,7 +672,8 @@ __global__ void __launch_bounds__(256) k_gather(
         // From the previous sorted key (bits from the top: tile, support
         // rank, position, plane, sub-cell table): the same tile and
         // position, the same plane and sub-cell table (hence the same table
-        // reference).
+        // reference).  The previous point ends the previous run, whose
+        // record carries its key.
         unsigned flags = kNewPos | kNewTab;
         if (s > 0)
         {
@@ -639,24 +681,46 @@ __global__ void __launch_bounds__(256) k_gather(
             if ((d >> pos_shift) == 0) flags &= ~kNewPos;
             if ((d & ((1ull << pos_shift) - 1)) == 0) flags &= ~kNewTab;
         }
-        meta[s] = make_int4(region_offset(r.pos, R, tmask), (int)tref,
+        meta[m] = make_int4(region_offset(r.pos, R, tmask), (int)tref,
                 (int)((unsigned)plane | ((unsigned)pi.srank << 16) | flags),
                 gw ? (int)min(gw[i], 0x7FFFFFFFu) : 0);
-        typename Types<T>::T4 q;
-        q.x = r.re;
-        q.y = -sgn * r.im;
-        q.z = r.im;
-        q.w = sgn * r.re;
-        vq[s] = q;
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        auto add = [&](const Rec& p) {
+            const double sgn = ((p.sub >> 16) & 1) ? -1.0 : 1.0;
+            q0 += (double)p.re;
+            q1 += -sgn * (double)p.im;
+            q2 += (double)p.im;
+            q3 += sgn * (double)p.re;
+        };
+        add(r);
+        // The rest of the run four points at a time: independent reads.
+        unsigned t = s + 1;
+        for (; t + 4 <= e; t += 4)
+        {
+            const unsigned i0 = idx[t], i1 = idx[t + 1], i2 = idx[t + 2],
+                    i3 = idx[t + 3];
+            const Rec p0 = rec[i0], p1 = rec[i1], p2 = rec[i2], p3 = rec[i3];
+            add(p0);
+            add(p1);
+            add(p2);
+            add(p3);
+        }
+        for (; t < e; ++t) add(rec[idx[t]]);
+        typename Types<T>::T4 o;
+        o.x = (T)q0;
+        o.y = (T)q1;
+        o.z = (T)q2;
+        o.w = (T)q3;
+        vq[m] = o;
     }
 }
 
-// bstart[b] = first sorted point of tile bucket b; bstart[n_buckets] = nv.
+// bstart[b] = first record of tile bucket b; bstart[n_buckets] = records.
 __global__ void __launch_bounds__(256) k_buckets(
         const Summary* __restrict__ summary, const unsigned* __restrict__ head,
         const unsigned* __restrict__ bpos, unsigned* __restrict__ bstart)
 {
-    const unsigned nv = summary->nv32;
+    const unsigned nv = summary->nm32;
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
          s += gridDim.x * blockDim.x)
         if (head[s]) bstart[bpos[s]] = s;
@@ -2405,16 +2469,30 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
             (const double*)d_normtab_, d_summary_);
     const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
             (n + 255) / 256, (size_t)n_cu_ * 4));
+    // Runs of equal keys -> one record each (not for the diagnostics, which
+    // count points per cell): run heads into the sort's spent value buffer,
+    // their scan into bpos (free until the tile scan below), the runs' keys
+    // into the sort's spent key buffer.
+    unsigned* const mh = d_idx_[0];
+    unsigned long long* const mkeys = d_keys_[0];
+    hipLaunchKernelGGL(k_merge_heads, dim3(blocks), dim3(256), 0, stream_,
+            (const Summary*)d_summary_, (const unsigned long long*)d_keys_[1],
+            mh, diag ? 0 : 1);
+    scan_u32(mh, d_bpos_, &d_summary_->nv32, n, &d_summary_->nm32);
+    // Run starts into the bucket table (free until the tile scan below).
+    hipLaunchKernelGGL(k_run_starts, dim3(blocks), dim3(256), 0, stream_,
+            (const Summary*)d_summary_, (const unsigned*)mh,
+            (const unsigned*)d_bpos_, d_bstart_);
     hipLaunchKernelGGL((k_gather<T>), dim3(blocks), dim3(256), 0, stream_,
             (const Summary*)d_summary_,
             (const unsigned long long*)d_keys_[1], (const unsigned*)d_idx_[1],
             (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
             (const PlaneInfo*)d_pinfo_, h_, K_, tile_shift_, d_meta_,
             (T4*)d_vq_, d_head_, diag ? 0 : region_, (1 << tile_bits_) - 1,
-            combo_bits_ + plane_bits_);
+            combo_bits_ + plane_bits_, (const unsigned*)d_bstart_, mkeys);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_gather launch");
     // Tiles: exclusive scan of the heads, then their starts.
-    scan_u32(d_head_, d_bpos_, &d_summary_->nv32, n, &d_summary_->n_buckets);
+    scan_u32(d_head_, d_bpos_, &d_summary_->nm32, n, &d_summary_->n_buckets);
     hipLaunchKernelGGL(k_buckets, dim3(blocks), dim3(256), 0, stream_,
             (const Summary*)d_summary_, (const unsigned*)d_head_,
             (const unsigned*)d_bpos_, d_bstart_);
@@ -2426,7 +2504,7 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
             &d_summary_->total_items);
     hipLaunchKernelGGL(k_write_items, dim3(blocks), dim3(256), 0, stream_,
             d_summary_,
-            (const unsigned long long*)d_keys_[1], (const unsigned*)d_bstart_,
+            (const unsigned long long*)mkeys, (const unsigned*)d_bstart_,
             (const unsigned*)d_bpos_, tile_bits_, tile_shift_,
             (unsigned)kJobPoints, (unsigned)items_cap_, d_items_);
     check_hip(hipGetLastError(), kErrKernelLaunch, "job launch");
@@ -2695,22 +2773,36 @@ int radix_sort_enqueue(size_t n, int bits, unsigned long long* keys[2],
 
 // Tile ranges of the sorted keys: tstart[t] = first point, tend[t] = one
 // past the last (both zero for empty tiles); bsum->nv32 = points gridded.
-__global__ void __launch_bounds__(256) k_tile_bounds(size_t n,
+// Sorted points gridded (all but the skipped ones, whose keys carry the
+// sentinel tile and sort last) -> bsum->nv32.
+__global__ void __launch_bounds__(256) k_batch_count(size_t n,
         const unsigned long long* __restrict__ keys, int lmax,
-        unsigned sentinel_tile, unsigned* __restrict__ tstart,
-        unsigned* __restrict__ tend, Summary* __restrict__ bsum)
+        unsigned sentinel_tile, Summary* __restrict__ bsum)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
     {
+        if ((unsigned)(keys[i] >> lmax) == sentinel_tile) continue;
+        if (i + 1 == n || (unsigned)(keys[i + 1] >> lmax) == sentinel_tile)
+            bsum->nv32 = (unsigned)(i + 1);
+    }
+}
+
+// Record range [tstart[t], tend[t]) of every global tile t that has
+// records (the bsum->nm32 run records, in key order).
+__global__ void __launch_bounds__(256) k_tile_bounds(
+        const unsigned long long* __restrict__ keys, int lmax,
+        unsigned* __restrict__ tstart, unsigned* __restrict__ tend,
+        const Summary* __restrict__ bsum)
+{
+    const unsigned nm = bsum->nm32;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nm;
+         i += gridDim.x * blockDim.x)
+    {
         const unsigned t = (unsigned)(keys[i] >> lmax);
-        if (t == sentinel_tile) continue;
-        if (i == 0 || (unsigned)(keys[i - 1] >> lmax) != t)
-            tstart[t] = (unsigned)i;
-        const unsigned nt = i + 1 < n ? (unsigned)(keys[i + 1] >> lmax)
-                                      : sentinel_tile;
-        if (nt != t) tend[t] = (unsigned)(i + 1);
-        if (nt == sentinel_tile) bsum->nv32 = (unsigned)(i + 1);
+        if (i == 0 || (unsigned)(keys[i - 1] >> lmax) != t) tstart[t] = i;
+        if (i + 1 == nm || (unsigned)(keys[i + 1] >> lmax) != t)
+            tend[t] = i + 1;
     }
 }
 
@@ -2765,25 +2857,30 @@ __global__ void __launch_bounds__(256) k_write_tile_items(unsigned ntiles,
     }
 }
 
-// Sorted point records of all jobs (as k_gather, with each point's job).
+// Run records of all jobs (as k_gather, with each record's job): run m =
+// sorted points [rstart[m], rstart[m + 1]), its visibility terms summed in
+// FP64; mkeys[m] = the run's key.
 __global__ void __launch_bounds__(256) k_gather_batch(
         const Summary* __restrict__ bsum,
         const unsigned long long* __restrict__ keys,
         const unsigned* __restrict__ idx, const RecF* __restrict__ rec,
         const BatchJob* __restrict__ jobs, int njobs, int lmax,
-        int4* __restrict__ meta, float4* __restrict__ vq)
+        int4* __restrict__ meta, float4* __restrict__ vq,
+        const unsigned* __restrict__ rstart,
+        unsigned long long* __restrict__ mkeys)
 {
-    const unsigned nv = bsum->nv32;
-    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
-         s += gridDim.x * blockDim.x)
+    const unsigned nm = bsum->nm32;
+    for (unsigned m = blockIdx.x * blockDim.x + threadIdx.x; m < nm;
+         m += gridDim.x * blockDim.x)
     {
+        const unsigned s = rstart[m], e = rstart[m + 1];
         const unsigned long long key = keys[s];
+        mkeys[m] = key;
         const int jd = batch_job_of_tile(jobs, njobs,
                 (unsigned)(key >> lmax));
         const BatchJob& d = jobs[jd];
         const RecF r = rec[idx[s]];
         const int plane = (unsigned)r.sub >> 17;
-        const float sgn = ((r.sub >> 16) & 1) ? -1.0f : 1.0f;
         const int off_u = (r.sub & 0xFF) - 128;
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
         const PlaneInfo& pi = d.pinfo[plane];
@@ -2798,11 +2895,33 @@ __global__ void __launch_bounds__(256) k_gather_batch(
             if ((dk >> d.pos_shift) == 0) flags &= ~kNewPos;
             if ((dk & ((1ull << d.pos_shift) - 1)) == 0) flags &= ~kNewTab;
         }
-        meta[s] = make_int4(region_offset(r.pos, d.R,
+        meta[m] = make_int4(region_offset(r.pos, d.R,
                 (1 << d.tile_bits) - 1), (int)tref,
                 (int)((unsigned)plane | ((unsigned)pi.srank << 16) | flags),
                 0);
-        vq[s] = make_float4(r.re, -sgn * r.im, r.im, sgn * r.re);
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        auto add = [&](const RecF& p) {
+            const double sgn = ((p.sub >> 16) & 1) ? -1.0 : 1.0;
+            q0 += (double)p.re;
+            q1 += -sgn * (double)p.im;
+            q2 += (double)p.im;
+            q3 += sgn * (double)p.re;
+        };
+        add(r);
+        unsigned t = s + 1;
+        for (; t + 4 <= e; t += 4)
+        {
+            const unsigned i0 = idx[t], i1 = idx[t + 1], i2 = idx[t + 2],
+                    i3 = idx[t + 3];
+            const RecF p0 = rec[i0], p1 = rec[i1], p2 = rec[i2],
+                    p3 = rec[i3];
+            add(p0);
+            add(p1);
+            add(p2);
+            add(p3);
+        }
+        for (; t < e; ++t) add(rec[idx[t]]);
+        vq[m] = make_float4((float)q0, (float)q1, (float)q2, (float)q3);
     }
 }
 
@@ -2904,7 +3023,8 @@ void Batch::free_buffers()
                     (void*)d_idx_[1], d_rec_, (void*)d_meta_, d_vq_,
                     (void*)d_tstart_, (void*)d_tend_, (void*)d_icnt_,
                     (void*)d_ioff_, (void*)d_part_, (void*)d_rhist_,
-                    (void*)d_roff_, (void*)d_items_})
+                    (void*)d_roff_, (void*)d_items_, (void*)d_mpos_,
+                    (void*)d_rstart_})
         dfree(p);
     if (h_sum_) (void)hipHostFree(h_sum_);
     d_desc_ = nullptr;
@@ -2917,6 +3037,7 @@ void Batch::free_buffers()
     d_vq_ = nullptr;
     d_tstart_ = d_tend_ = d_icnt_ = d_ioff_ = d_part_ = nullptr;
     d_rhist_ = d_roff_ = nullptr;
+    d_mpos_ = d_rstart_ = nullptr;
     d_items_ = nullptr;
     h_sum_ = nullptr;
     cap_points_ = cap_tiles_ = cap_combo_ = cap_jobs_ = 0;
@@ -3033,6 +3154,8 @@ void Batch::prepare()
         d_vq_ = dmalloc<float4>(cap_points_);
         d_tstart_ = dmalloc<unsigned>(cap_tiles_);
         d_tend_ = dmalloc<unsigned>(cap_tiles_);
+        d_mpos_ = dmalloc<unsigned>(cap_points_);
+        d_rstart_ = dmalloc<unsigned>(cap_points_ + 1);
         const size_t rh = radix_hist_words(cap_points_);
         d_rhist_ = dmalloc<unsigned>(rh);
         d_roff_ = dmalloc<unsigned>(rh);
@@ -3156,9 +3279,27 @@ void Batch::enqueue()
             d_rhist_, d_roff_, d_part_, d_ntiles_ + 1, &d_bsum_->n_buckets, s);
     const unsigned long long* kin = kk[cur];
     const unsigned* vin = vv[cur];
+    // Runs of equal keys (same job, tile, origin, plane and sub-cell table)
+    // -> one record each: run heads into the sort's spent value buffer, the
+    // runs' keys into its spent key buffer; tiles over the run records.
+    unsigned* const mh = vv[1 - cur];
+    unsigned long long* const mkeys = kk[1 - cur];
+    hipLaunchKernelGGL(k_batch_count, dim3(pblocks), dim3(256), 0, s,
+            n_total_, kin, lmax_, (unsigned)tiles_total_, d_bsum_);
+    hipLaunchKernelGGL(k_merge_heads, dim3(pblocks), dim3(256), 0, s,
+            (const Summary*)d_bsum_, kin, mh, 1);
+    scan_u32_enqueue(mh, d_mpos_, &d_bsum_->nv32, n_total_, &d_bsum_->nm32,
+            d_part_, s);
+    hipLaunchKernelGGL(k_run_starts, dim3(pblocks), dim3(256), 0, s,
+            (const Summary*)d_bsum_, (const unsigned*)mh,
+            (const unsigned*)d_mpos_, d_rstart_);
+    hipLaunchKernelGGL(k_gather_batch, dim3(pblocks), dim3(256), 0, s,
+            (const Summary*)d_bsum_, kin, vin, (const RecF*)d_rec_,
+            (const BatchJob*)d_desc_, (int)J, lmax_, d_meta_,
+            (float4*)d_vq_, (const unsigned*)d_rstart_, mkeys);
     hipLaunchKernelGGL(k_tile_bounds, dim3(pblocks), dim3(256), 0, s,
-            n_total_, (const unsigned long long*)kin, lmax_,
-            (unsigned)tiles_total_, d_tstart_, d_tend_, d_bsum_);
+            (const unsigned long long*)mkeys, lmax_, d_tstart_, d_tend_,
+            (const Summary*)d_bsum_);
     hipLaunchKernelGGL(k_tile_items, dim3(tblocks), dim3(256), 0, s,
             (const unsigned*)d_ntiles_, (const unsigned*)d_tstart_,
             (const unsigned*)d_tend_, d_icnt_);
@@ -3168,11 +3309,6 @@ void Batch::enqueue()
             (unsigned)tiles_total_, (const unsigned*)d_tstart_,
             (const unsigned*)d_tend_, (const unsigned*)d_ioff_,
             (const BatchJob*)d_desc_, (int)J, (unsigned)items_cap_, d_items_);
-    hipLaunchKernelGGL(k_gather_batch, dim3(pblocks), dim3(256), 0, s,
-            (const Summary*)d_bsum_, (const unsigned long long*)kin,
-            (const unsigned*)vin, (const RecF*)d_rec_,
-            (const BatchJob*)d_desc_, (int)J, lmax_, d_meta_,
-            (float4*)d_vq_);
     hipLaunchKernelGGL(k_grid_batch, dim3(grid_blocks_), dim3(kGridThreads),
             0, s, d_bsum_, (const int4*)d_items_,
             (const BatchJob*)d_desc_, (const int4*)d_meta_,

@@ -76,6 +76,8 @@ struct Summary
     unsigned flags;                 // bit 0: diagnostics plane index beyond
                                     // its table; bit 1: work items overflow
     unsigned item_next[kXcdGroups]; // scatter queue heads, one per XCD
+    unsigned nm32;                  // scatter records: runs of equal sort
+                                    // keys (nv32 without merging)
 };
 
 // Per-point record written by the locate kernel in input order.
@@ -470,8 +472,11 @@ private:
     void* d_rec_ = nullptr;
     int4* d_meta_ = nullptr;
     void* d_vq_ = nullptr;
-    unsigned* d_tstart_ = nullptr;    // per global tile: first point
+    unsigned* d_tstart_ = nullptr;    // per global tile: first record
     unsigned* d_tend_ = nullptr;      // one past the last
+    unsigned* d_mpos_ = nullptr;      // per sorted point: index of its run
+                                      // (runs of equal keys: one record)
+    unsigned* d_rstart_ = nullptr;    // per run: first sorted point
     unsigned* d_icnt_ = nullptr;
     unsigned* d_ioff_ = nullptr;
     unsigned* d_part_ = nullptr;
