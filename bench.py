@@ -715,6 +715,7 @@ def main():
         stage_ms.append(plan.timings())
     fg.device_synchronize()
     wall = time.perf_counter() - t0
+    records, record_updates = plan.records()
 
     n = wl.num_points
     ms_per_step = wall / args.steps * 1e3
@@ -769,7 +770,17 @@ def main():
                      "kernel": "k_grid<float> (scatter)",
                      "compute": "FP32 VALU FMA, no MFMA; peak is MI355X "
                                 "FP32 157.3 TFLOP/s",
-                     "flops_per_launch": flops},
+                     "flops_per_launch": flops,
+                     "flops": "algorithmic: 8 per grid update of the "
+                              "reference loop, point by point"},
+        "executed": {"records": int(records),
+                     "grid_updates": int(record_updates),
+                     "frac_of_algorithmic": record_updates / updates
+                     if updates else None,
+                     "tflops": 8.0 * record_updates / (grid_ms * 1e-3) / 1e12,
+                     "what": "the scatter takes one record per run of "
+                             "points with equal origin, w-plane and "
+                             "sub-cell table (visibility terms summed)"},
     }
     if not args.no_variants:
         out["precision_variants"] = precision_variants(wl, d_uvw, d_vis,

@@ -35,7 +35,8 @@ EXPORTED_SYMBOLS = (
     "gw_release_cached",
     "gw_plan_create", "gw_plan_free", "gw_plan_set_stream", "gw_grid_device",
     "gw_grid_device_noalloc",
-    "gw_plan_results", "gw_plan_timings", "gw_plan_set_tuning",
+    "gw_plan_results", "gw_plan_records", "gw_plan_timings",
+    "gw_plan_set_tuning",
     "gw_plan_get_tuning", "gw_plan_autotune",
     "gw_device_count", "gw_device_malloc", "gw_device_free",
     "gw_host_malloc", "gw_host_free",
@@ -142,6 +143,9 @@ def lib():
     L.gw_plan_results.argtypes = [_vp, _szp, _dp,
                                   ct.POINTER(ct.c_ulonglong), _ip]
     L.gw_plan_results.restype = None
+    L.gw_plan_records.argtypes = [_vp, ct.POINTER(ct.c_ulonglong),
+                                  ct.POINTER(ct.c_ulonglong), _ip]
+    L.gw_plan_records.restype = None
     L.gw_plan_timings.argtypes = [_vp, _dp, ct.c_int]
     L.gw_plan_timings.restype = ct.c_int
     L.gw_plan_set_tuning.argtypes = [_vp, ct.c_int, ct.c_int, ct.c_int, _ip]
@@ -703,6 +707,18 @@ class Plan:
                               ct.byref(nu), ct.byref(st))
         _check(st, "gw_plan_results")
         return int(sk.value), float(nd.value), int(nu.value)
+
+    def records(self):
+        """(records, record_updates) the last grid() call's scatter took:
+        one record per run of points with equal origin, plane and sub-cell
+        table, and the taps it applied."""
+        st = ct.c_int(0)
+        nr = ct.c_ulonglong(0)
+        nu = ct.c_ulonglong(0)
+        lib().gw_plan_records(self.ptr, ct.byref(nr), ct.byref(nu),
+                              ct.byref(st))
+        _check(st, "gw_plan_records")
+        return int(nr.value), int(nu.value)
 
     def timings(self):
         buf = (ct.c_double * 8)()

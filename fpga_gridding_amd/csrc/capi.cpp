@@ -1073,6 +1073,21 @@ void gw_plan_results(gw_plan* plan, size_t* num_skipped, double* norm_delta,
     });
 }
 
+GW_EXPORT
+void gw_plan_records(gw_plan* plan, unsigned long long* num_records,
+        unsigned long long* record_updates, int* status)
+{
+    int local = 0;
+    if (!status) status = &local;
+    if (*status) return;
+    guarded(status, [&] {
+        if (!plan) gw::fail(gw::kErrInvalidArgument, "plan is NULL");
+        const gw::Summary& sm = plan->impl->read_summary();
+        if (num_records) *num_records = sm.nm32;
+        if (record_updates) *record_updates = sm.run_updates;
+    });
+}
+
 GW_EXPORT int gw_plan_timings(gw_plan* plan, double* ms, int max_n)
 {
     if (!plan || !ms) return 0;

@@ -227,6 +227,8 @@ __global__ void k_init(Summary* __restrict__ s, unsigned* __restrict__ combo,
         s->total_items = 0;
         s->flags = 0;
         for (int q = 0; q < kXcdGroups; ++q) s->item_next[q] = 0;
+        s->nm32 = 0;
+        s->run_updates = 0;
     }
     for (int i = t; i < n_combo; i += gridDim.x * blockDim.x) combo[i] = 0u;
 }
@@ -639,10 +641,11 @@ __global__ void __launch_bounds__(256) k_run_starts(
 // component: each point's (re, -s*im, im, s*re) applies its own
 // conjugation sign, so the sum is exact algebra for any mix of signs
 // (summed in FP64 in sorted order, then stored in T).  mkeys[m] = the
-// run's key.
+// run's key; summary->run_updates = the footprint cells of all runs (the
+// grid updates the scatter executes).
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather(
-        const Summary* __restrict__ summary,
+        Summary* __restrict__ summary,
         const unsigned long long* __restrict__ keys,
         const unsigned* __restrict__ idx,
         const typename Types<T>::Rec* __restrict__ rec,
@@ -654,6 +657,7 @@ __global__ void __launch_bounds__(256) k_gather(
 {
     typedef typename Types<T>::Rec Rec;
     const unsigned nm = summary->nm32;
+    unsigned long long upd = 0;     // footprint cells of this thread's runs
     for (unsigned m = blockIdx.x * blockDim.x + threadIdx.x; m < nm;
          m += gridDim.x * blockDim.x)
     {
@@ -669,6 +673,7 @@ __global__ void __launch_bounds__(256) k_gather(
         const int off_v = ((r.sub >> 8) & 0xFF) - 128;
         const PlaneInfo& pi = pinfo[plane];
         const unsigned tref = table_ref(pi, off_u, off_v, h);
+        upd += (unsigned long long)pi.W * (unsigned long long)pi.W;
         // From the previous sorted key (bits from the top: tile, support
         // rank, position, plane, sub-cell table): the same tile and
         // position, the same plane and sub-cell table (hence the same table
@@ -713,6 +718,9 @@ __global__ void __launch_bounds__(256) k_gather(
         o.w = (T)q3;
         vq[m] = o;
     }
+    upd = wave_sum(upd);
+    if ((threadIdx.x & 63) == 0 && upd)
+        atomicAdd(&summary->run_updates, upd);
 }
 
 // bstart[b] = first record of tile bucket b; bstart[n_buckets] = records.
@@ -2484,7 +2492,7 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
             (const Summary*)d_summary_, (const unsigned*)mh,
             (const unsigned*)d_bpos_, d_bstart_);
     hipLaunchKernelGGL((k_gather<T>), dim3(blocks), dim3(256), 0, stream_,
-            (const Summary*)d_summary_,
+            d_summary_,
             (const unsigned long long*)d_keys_[1], (const unsigned*)d_idx_[1],
             (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
             (const PlaneInfo*)d_pinfo_, h_, K_, tile_shift_, d_meta_,

@@ -78,6 +78,8 @@ struct Summary
     unsigned item_next[kXcdGroups]; // scatter queue heads, one per XCD
     unsigned nm32;                  // scatter records: runs of equal sort
                                     // keys (nv32 without merging)
+    unsigned long long run_updates; // sum over records of W^2: the grid
+                                    // updates the scatter executes
 };
 
 // Per-point record written by the locate kernel in input order.

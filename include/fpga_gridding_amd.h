@@ -187,6 +187,17 @@ GW_EXPORT
 void gw_plan_results(gw_plan* plan, size_t* num_skipped, double* norm_delta,
         unsigned long long* num_updates, int* status);
 
+/* Work the scatter of the plan's last gw_grid_device call executed (no
+ * counterpart in the reference; a measurement aid): points whose sort keys
+ * are equal (same footprint origin, w-plane and sub-cell table) are
+ * gridded as one record with their visibility terms summed, so the
+ * scatter takes num_records records and applies record_updates taps (sum
+ * over records of (2*support+1)^2), against gw_plan_results' num_updates
+ * for the points one by one.  Waits for the call like gw_plan_results. */
+GW_EXPORT
+void gw_plan_records(gw_plan* plan, unsigned long long* num_records,
+        unsigned long long* record_updates, int* status);
+
 /* Scatter shape of a plan (no counterpart in the reference; a tuning knob,
  * results do not depend on it): tile side 2^tile_log2 of footprint origins
  * per work item (-1: the largest whose FP64 LDS region fits), at most

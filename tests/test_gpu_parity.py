@@ -125,6 +125,90 @@ def test_shapes_fp64(case):
     assert max_rel(g, gd) <= 1e-12
 
 
+def _runs_workload(seed=11):
+    """Points repeated in runs that share cell, plane and sub-cell table:
+    each base point appears as itself, an exact copy, with w negated (same
+    plane, conjugated kernel) and as a copy of that, in shuffled order and
+    with its own visibilities, so merged runs mix conjugation signs."""
+    base = synth.random_small(n=400, grid_size=128, num_planes=8, s_min=2,
+                              s_max=12, ov=4, seed=seed, edge_fraction=0.05)
+    rng = np.random.default_rng(seed)
+    reps = 4
+    uu = np.tile(base.uu, reps)
+    vv = np.tile(base.vv, reps)
+    ww = np.concatenate([base.ww, base.ww, -base.ww, -base.ww])
+    order = rng.permutation(uu.size)
+    n = uu.size
+    return dataclasses.replace(
+        base, uu=uu[order], vv=vv[order], ww=ww[order].astype(np.float32),
+        vis=rng.standard_normal(2 * n).astype(np.float32),
+        weight=np.ones(n, np.float32), name="runs")
+
+
+def test_merged_runs_mixed_conjugation():
+    """Runs of points with equal cell, plane and sub-cell table become one
+    scatter record whose visibility terms are summed with each point's own
+    conjugation sign: FP32 to the FP32 bar, FP64 to 1e-12 of the FP64
+    oracle, skip counts exact, the batch path to the same bars."""
+    wl = _runs_workload()
+    g, sk, nrm = fg.grid_workload_f(wl)
+    r = fp32_report(g, wl)
+    assert sk == r["skipped_f"]
+    assert abs(nrm - r["norm_f"]) <= 1e-12 * abs(r["norm_f"])
+    assert_fp32_parity(r)
+    gd, skd, nd = fg.grid_workload_d(wl)
+    g_o, sk_o, n_o = oracle.grid_wproj_d(wl)
+    assert skd == sk_o
+    assert abs(nd - n_o) <= 1e-12 * abs(n_o)
+    assert max_rel(gd, g_o) <= 1e-12
+    # The plan API reports one record per run: the distinct (cell, plane,
+    # sub-cell) of the gridded points, and their footprint sizes.
+    plan = fg.Plan.for_workload(wl, fg.GW_SINGLE)
+    G = wl.grid_size
+    d = [fg.DeviceArray.from_host(a) for a in (wl.uu, wl.vv, wl.ww, wl.vis)]
+    dg = fg.DeviceArray(8 * G * G)
+    dg.zero()
+    plan.grid(wl.num_points, *d, dg)
+    skp, _, upd = plan.results()
+    nrec, nrec_upd = plan.records()
+    rec, u_alg = oracle.locate_f(wl)
+    ok = rec["skip"] == 0
+    keys = np.stack([rec[k][ok] for k in ("grid_u", "grid_v", "plane",
+                                          "off_u", "off_v")], axis=1)
+    uniq, first = np.unique(keys, axis=0, return_index=True)
+    w = 2 * rec["support"][ok][first].astype(np.int64) + 1
+    assert skp == sk and upd == u_alg
+    assert nrec == len(uniq) and nrec_upd == int((w * w).sum())
+    assert 4 * nrec == ok.sum()
+    for x in (*d, dg):
+        x.close()
+    # The same points through a two-job batch (FP32 index math).
+    jobs = []
+    for part in (slice(0, 700), slice(700, None)):
+        sub = dataclasses.replace(wl, uu=wl.uu[part], vv=wl.vv[part],
+                                  ww=wl.ww[part],
+                                  vis=wl.vis.reshape(-1, 2)[part].ravel())
+        d = [fg.DeviceArray.from_host(a) for a in (sub.uu, sub.vv, sub.ww,
+                                                    sub.vis)]
+        dg = fg.DeviceArray(8 * G * G)
+        dg.zero()
+        jobs.append((sub, d, dg))
+    batch = fg.Batch()
+    for sub, d, dg in jobs:
+        batch.add(plan, sub.num_points, *d, dg)
+    batch.run()
+    res = [batch.results(j) for j in range(len(jobs))]
+    for (sub, d, dg), (skb, nb, _) in zip(jobs, res):
+        gb = np.empty(2 * G * G, np.float32)
+        dg.download(gb)
+        rb = fp32_report(gb, sub)
+        assert skb == rb["skipped_f"]
+        assert abs(nb - rb["norm_f"]) <= 1e-12 * abs(rb["norm_f"])
+        assert_fp32_parity(rb)
+    batch.close()
+    plan.close()
+
+
 def test_plane_index_ties():
     """w placed on and one ulp beside the plane rounding ties
     sqrt(|w| w_scale) = m + 1/2: any device/host difference in sqrt or
