@@ -732,11 +732,12 @@ class Plan:
         _check(st, "gw_plan_set_tuning")
 
     def tuning(self):
-        """dict(tile_log2, slot_cap, min_share, region, max_tile_log2)."""
-        buf = (ct.c_int * 5)()
-        n = lib().gw_plan_get_tuning(self.ptr, buf, 5)
+        """dict(tile_log2 (-1 for a side that is not a power of two),
+        slot_cap, min_share, region, max_tile_log2, tile, max_tile)."""
+        buf = (ct.c_int * 7)()
+        n = lib().gw_plan_get_tuning(self.ptr, buf, 7)
         keys = ("tile_log2", "slot_cap", "min_share", "region",
-                "max_tile_log2")
+                "max_tile_log2", "tile", "max_tile")
         return dict(zip(keys[:n], list(buf[:n])))
 
     def autotune(self, n, d_uu, d_vv, d_ww, d_vis, d_scratch, reps=2):

@@ -1028,10 +1028,12 @@ GW_EXPORT
 int gw_plan_get_tuning(gw_plan* plan, int* values, int max_n)
 {
     if (!plan || !values || max_n <= 0) return 0;
-    const int v[5] = {plan->impl->tile_bits(), plan->impl->slot_cap(),
-                      plan->impl->min_share(), plan->impl->region(),
-                      plan->impl->max_tile_bits()};
-    const int k = max_n < 5 ? max_n : 5;
+    const gw::Plan& p = *plan->impl;
+    const int t = p.tile();
+    const int v[7] = {(t & (t - 1)) == 0 ? p.tile_bits() : -1, p.slot_cap(),
+                      p.min_share(), p.region(), p.max_tile_bits(), t,
+                      p.max_tile()};
+    const int k = max_n < 7 ? max_n : 7;
     for (int i = 0; i < k; ++i) values[i] = v[i];
     return k;
 }

@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(256) k_locate(
         const TU* __restrict__ ww, const TV* __restrict__ vis, T grid_scale,
         T w_scale, int ov, int G, int P, const int* __restrict__ support,
         const int* __restrict__ srank, int srank_bits,
-        int h, int K, int tile_bits, int combo_bits, int key_bits,
+        int h, int K, int tile, int tile_bits, int combo_bits, int key_bits,
         unsigned long long* __restrict__ keys, unsigned* __restrict__ idx,
         Rec* __restrict__ rec,
         unsigned* __restrict__ combo, Summary* __restrict__ summary,
@@ -335,11 +335,20 @@ __global__ void __launch_bounds__(256) k_locate(
             const int iv = min(max(off_v + h, 0), K - 1);
             const int table = iv * K + iu;
             const int c = plane * K * K + table;
-            const unsigned long long mort = spread_bits16((unsigned)u0) |
-                    (spread_bits16((unsigned)v0) << 1);
-            key = ((mort >> (2 * tile_bits)) << sh_tile) |
+            // Tile (tu, tv) of side `tile` and the origin's place (ou, ov)
+            // in it, each Z-ordered (for a power-of-two side, the origin's
+            // own Z-order split at 2 tile_bits).
+            const unsigned tu = (unsigned)u0 / (unsigned)tile;
+            const unsigned tv = (unsigned)v0 / (unsigned)tile;
+            const unsigned ou = (unsigned)u0 - tu * (unsigned)tile;
+            const unsigned ov = (unsigned)v0 - tv * (unsigned)tile;
+            const unsigned long long tz = spread_bits16(tu) |
+                    (spread_bits16(tv) << 1);
+            const unsigned long long pz = spread_bits16(ou) |
+                    (spread_bits16(ov) << 1);
+            key = (tz << sh_tile) |
                     ((unsigned long long)srank[plane] << sh_srank) |
-                    ((mort & ((1ull << (2 * tile_bits)) - 1ull)) << sh_pos) |
+                    (pz << sh_pos) |
                     ((unsigned long long)plane << sh_plane) |
                     (unsigned long long)table;
             if (lds_hist) atomicAdd(&s_combo[c], 1u);
@@ -598,12 +607,13 @@ constexpr unsigned kRankMask = 0x3FFFu;
 // meta.x of a point for the LDS scatter: its origin's byte offset in the
 // region of its tile (tiles start at multiples of the tile side T, region
 // pitch R cells, FP64 cells); R == 0: the origin u | v << 16 itself.
-__device__ __forceinline__ int region_offset(int pos, int R, int tmask)
+__device__ __forceinline__ int region_offset(int pos, int R, int tile)
 {
     if (R <= 0) return pos;
-    const int u = pos & 0xFFFF;
-    const int v = (int)((unsigned)pos >> 16);
-    return ((v & tmask) * R + (u & tmask)) * (int)sizeof(double);
+    const unsigned u = (unsigned)pos & 0xFFFFu;
+    const unsigned v = (unsigned)pos >> 16;
+    return ((int)(v % (unsigned)tile) * R + (int)(u % (unsigned)tile)) *
+            (int)sizeof(double);
 }
 
 // Runs of equal sort keys: mh[s] = 1 where sorted point s starts one (its
@@ -652,7 +662,7 @@ __global__ void __launch_bounds__(256) k_gather(
         const unsigned* __restrict__ gw, const PlaneInfo* __restrict__ pinfo,
         int h, int K, int tile_shift, int4* __restrict__ meta,
         typename Types<T>::T4* __restrict__ vq, unsigned* __restrict__ head,
-        int R, int tmask, int pos_shift, const unsigned* __restrict__ rstart,
+        int R, int tile, int pos_shift, const unsigned* __restrict__ rstart,
         unsigned long long* __restrict__ mkeys)
 {
     typedef typename Types<T>::Rec Rec;
@@ -686,7 +696,7 @@ __global__ void __launch_bounds__(256) k_gather(
             if ((d >> pos_shift) == 0) flags &= ~kNewPos;
             if ((d & ((1ull << pos_shift) - 1)) == 0) flags &= ~kNewTab;
         }
-        meta[m] = make_int4(region_offset(r.pos, R, tmask), (int)tref,
+        meta[m] = make_int4(region_offset(r.pos, R, tile), (int)tref,
                 (int)((unsigned)plane | ((unsigned)pi.srank << 16) | flags),
                 gw ? (int)min(gw[i], 0x7FFFFFFFu) : 0);
         double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
@@ -764,7 +774,7 @@ __global__ void __launch_bounds__(256) k_write_items(
         Summary* __restrict__ summary,
         const unsigned long long* __restrict__ keys,
         const unsigned* __restrict__ bstart,
-        const unsigned* __restrict__ ioff, int tile_bits, int tile_shift,
+        const unsigned* __restrict__ ioff, int tile, int tile_shift,
         unsigned job_points, unsigned items_cap, int4* __restrict__ items)
 {
     if (blockIdx.x == 0 && threadIdx.x < kXcdGroups)
@@ -774,9 +784,9 @@ __global__ void __launch_bounds__(256) k_write_items(
          b += gridDim.x * blockDim.x)
     {
         const unsigned start = bstart[b], end = bstart[b + 1];
-        const unsigned long long tile = keys[start] >> tile_shift;
-        const unsigned ux0 = compact_bits16(tile) << tile_bits;
-        const unsigned vy0 = compact_bits16(tile >> 1) << tile_bits;
+        const unsigned long long tz = keys[start] >> tile_shift;
+        const unsigned ux0 = compact_bits16(tz) * (unsigned)tile;
+        const unsigned vy0 = compact_bits16(tz >> 1) * (unsigned)tile;
         const int origin = (int)(ux0 | (vy0 << 16));
         unsigned o = ioff[b];
         for (unsigned first = start; first < end; first += job_points, ++o)
@@ -1870,7 +1880,11 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
                 support_[p]) - svals.begin());
     srank_bits_ = std::max(1, bits_for((unsigned long long)(svals.size() -
             1)));
-    key_bits_ = 2 * coord_bits_ + combo_bits_ + plane_bits_ + srank_bits_;
+    // Tile and in-tile coordinates take at most one bit more per axis
+    // than a grid coordinate (one each for a side that is not a power of
+    // two); configure() sets the exact width for the chosen side.
+    key_bits_max_ = 2 * coord_bits_ + 2 + combo_bits_ + plane_bits_ +
+            srank_bits_;
     pinfo_.resize(P_);
     for (size_t p = 0; p < P_; ++p)
     {
@@ -2049,7 +2063,26 @@ int Plan::max_tile_bits() const
     return -1;
 }
 
+int Plan::max_tile() const
+{
+    const long long wmax = 2ll * s_max_ + 1;
+    for (int t = 1 << std::min(6, coord_bits_); t >= 1; --t)
+    {
+        const long long R = t + wmax - 1;
+        if (R * R + kSinkCells <= kRegionCells) return t;
+    }
+    return -1;
+}
+
 void Plan::configure(int tile_bits, int slot_cap, int min_share)
+{
+    if (tile_bits > std::min(6, coord_bits_))
+        fail(kErrInvalidArgument, "tile_log2 must be in [0, " +
+             std::to_string(std::min(6, coord_bits_)) + "]");
+    configure_side(tile_bits < 0 ? -1 : 1 << tile_bits, slot_cap, min_share);
+}
+
+void Plan::configure_side(int tile, int slot_cap, int min_share)
 {
     const int smax_all = prec_ == kPrecDouble ? slots_max<double>() :
             slots_max<float>();
@@ -2059,27 +2092,32 @@ void Plan::configure(int tile_bits, int slot_cap, int min_share)
     if (min_share < 0 || min_share > kJobPoints)
         fail(kErrInvalidArgument, "min_share must be in [0, " +
              std::to_string(kJobPoints) + "] (0: the default)");
-    // Tile side T = 2^t (<= 64) of footprint origins: by default the
-    // largest whose LDS region (T + Wmax - 1)^2 plus the sink cells fits
-    // each region array.  If none does the scatter adds straight into the
-    // grid in HBM.
-    const int tmax = max_tile_bits();
+    // Tile side T (<= 64) of footprint origins: by default the largest whose
+    // LDS region (T + Wmax - 1)^2 plus the sink cells fits each region
+    // array, a power of two or not (a side that fills the region makes
+    // fewer, fuller work items where points are sparse).  If none fits the
+    // scatter adds straight into the grid in HBM.
+    const int tcap = 1 << std::min(6, coord_bits_);
+    const int tmax = max_tile();
     const long long wmax = 2ll * s_max_ + 1;
-    if (tile_bits < 0)
+    if (tile < 0)
     {
-        tile_bits_ = tmax >= 0 ? tmax : std::min(6, coord_bits_);
-        region_ = tmax >= 0 ? (int)((1ll << tmax) + wmax - 1) : 0;
+        tile_ = tmax >= 1 ? tmax : tcap;
+        region_ = tmax >= 1 ? (int)(tmax + wmax - 1) : 0;
     }
     else
     {
-        if (tile_bits > std::min(6, coord_bits_))
-            fail(kErrInvalidArgument, "tile_log2 must be in [0, " +
-                 std::to_string(std::min(6, coord_bits_)) + "]");
-        tile_bits_ = tile_bits;
-        const long long R = (1ll << tile_bits) + wmax - 1;
+        if (tile < 1 || tile > tcap)
+            fail(kErrInvalidArgument, "tile side must be in [1, " +
+                 std::to_string(tcap) + "]");
+        tile_ = tile;
+        const long long R = tile + wmax - 1;
         region_ = R * R + kSinkCells <= kRegionCells ? (int)R : 0;
     }
+    tile_bits_ = bits_for((unsigned long long)(tile_ - 1));
+    tile_coord_bits_ = bits_for((unsigned long long)((G_ - 1) / tile_));
     tile_shift_ = 2 * tile_bits_ + combo_bits_ + plane_bits_ + srank_bits_;
+    key_bits_ = tile_shift_ + 2 * tile_coord_bits_;
     slot_cap_ = slot_cap > 0 ? slot_cap : smax_all;
     min_share_ = min_share > 0 ? min_share : kMinShare;
 
@@ -2137,11 +2175,11 @@ double Plan::autotune(size_t n, const void* uu, const void* vv,
     };
     const int smax_all = prec_ == kPrecDouble ? slots_max<double>() :
             slots_max<float>();
-    int bt = tile_bits_, bs = slot_cap_, bm = min_share_;
-    configure(bt, bs, bm);
+    int bt = tile_, bs = slot_cap_, bm = min_share_;
+    configure_side(bt, bs, bm);
     double best = measure();
     auto consider = [&](int t, int sc, int ms) {
-        configure(t, sc, ms);
+        configure_side(t, sc, ms);
         const double v = measure();
         if (v < best)
         {
@@ -2151,15 +2189,22 @@ double Plan::autotune(size_t n, const void* uu, const void* vv,
             bm = ms;
         }
     };
-    const int tmax = max_tile_bits();
-    if (tmax >= 0)
-        for (int t = tmax; t >= std::max(0, tmax - 2); --t)
-            if (t != bt) consider(t, bs, bm);
+    // Tile sides: the largest that fits, then the three largest powers of
+    // two not above it.
+    const int tfit = max_tile();
+    if (tfit >= 1)
+    {
+        int p2 = 1;
+        while (p2 * 2 <= tfit) p2 *= 2;
+        if (tfit != bt) consider(tfit, bs, bm);
+        for (int k = 0; k < 3 && p2 >= 1; ++k, p2 /= 2)
+            if (p2 != tfit && p2 != bt) consider(p2, bs, bm);
+    }
     for (int sc : {4, 6, 8})
         if (sc <= smax_all && sc != bs) consider(bt, sc, bm);
     for (int ms : {4, 8, 16})
         if (ms != bm) consider(bt, bs, ms);
-    configure(bt, bs, bm);
+    configure_side(bt, bs, bm);
     return best;
 }
 
@@ -2287,7 +2332,7 @@ size_t Plan::external_workspace_bytes(size_t n) const
     check_hip(rocprim::radix_sort_pairs(nullptr, sort_bytes,
             (unsigned long long*)nullptr, (unsigned long long*)nullptr,
             (unsigned*)nullptr, (unsigned*)nullptr, cap, 0u,
-            (unsigned)key_bits_ + 1u, stream_), kErrComputeDevices,
+            (unsigned)key_bits_max_ + 1u, stream_), kErrComputeDevices,
             "radix sort size query");
     Workspace w{};
     return carve_workspace(nullptr, cap, prec_ == kPrecDouble ?
@@ -2313,7 +2358,7 @@ void Plan::use_external_workspace(void* base, size_t bytes, size_t n)
     check_hip(rocprim::radix_sort_pairs(nullptr, w.sort_tmp_bytes,
             (unsigned long long*)nullptr, (unsigned long long*)nullptr,
             (unsigned*)nullptr, (unsigned*)nullptr, cap, 0u,
-            (unsigned)key_bits_ + 1u, stream_), kErrComputeDevices,
+            (unsigned)key_bits_max_ + 1u, stream_), kErrComputeDevices,
             "radix sort size query");
     carve_workspace(static_cast<char*>(base), cap, prec_ == kPrecDouble ?
             sizeof(RecD) : sizeof(RecF), prec_ == kPrecDouble ?
@@ -2388,7 +2433,7 @@ void Plan::ensure_capacity(size_t n)
     d_scan_part_ = dmalloc<unsigned>(cap / kScanTile + 2);
     size_t bytes = 0;
     check_hip(rocprim::radix_sort_pairs(nullptr, bytes, d_keys_[0],
-            d_keys_[1], d_idx_[0], d_idx_[1], cap, 0u, (unsigned)key_bits_ + 1u,
+            d_keys_[1], d_idx_[0], d_idx_[1], cap, 0u, (unsigned)key_bits_max_ + 1u,
             stream_), kErrComputeDevices, "radix sort size query");
     d_sort_tmp_ = dmalloc<char>(bytes);
     sort_tmp_bytes_ = bytes;
@@ -2440,7 +2485,7 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
         hipLaunchKernelGGL((k_locate<T, true, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
                 d_support_, (const int*)d_srank_, srank_bits_, h_, K_,
-                tile_bits_, combo_bits_, key_bits_,
+                tile_, tile_bits_, combo_bits_, key_bits_,
                 d_keys_[0], d_idx_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, d_gw_, diag->per_support,
                 diag->per_plane, diag->plane_cap, 0ull, 0u);
@@ -2448,7 +2493,7 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
         hipLaunchKernelGGL((k_locate<T, false, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
                 d_support_, (const int*)d_srank_, srank_bits_, h_, K_,
-                tile_bits_, combo_bits_, key_bits_,
+                tile_, tile_bits_, combo_bits_, key_bits_,
                 d_keys_[0], d_idx_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, (unsigned*)nullptr,
                 (unsigned long long*)nullptr, (unsigned long long*)nullptr,
@@ -2496,7 +2541,7 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
             (const unsigned long long*)d_keys_[1], (const unsigned*)d_idx_[1],
             (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
             (const PlaneInfo*)d_pinfo_, h_, K_, tile_shift_, d_meta_,
-            (T4*)d_vq_, d_head_, diag ? 0 : region_, (1 << tile_bits_) - 1,
+            (T4*)d_vq_, d_head_, diag ? 0 : region_, tile_,
             combo_bits_ + plane_bits_, (const unsigned*)d_bstart_, mkeys);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_gather launch");
     // Tiles: exclusive scan of the heads, then their starts.
@@ -2513,7 +2558,7 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     hipLaunchKernelGGL(k_write_items, dim3(blocks), dim3(256), 0, stream_,
             d_summary_,
             (const unsigned long long*)mkeys, (const unsigned*)d_bstart_,
-            (const unsigned*)d_bpos_, tile_bits_, tile_shift_,
+            (const unsigned*)d_bpos_, tile_, tile_shift_,
             (unsigned)kJobPoints, (unsigned)items_cap_, d_items_);
     check_hip(hipGetLastError(), kErrKernelLaunch, "job launch");
     check_hip(hipEventRecord(ev_[3], stream_), kErrKernelLaunch, "event");
@@ -2853,9 +2898,9 @@ __global__ void __launch_bounds__(256) k_write_tile_items(unsigned ntiles,
         const int j = batch_job_of_tile(jobs, njobs, t);
         const BatchJob& d = jobs[j];
         const unsigned lt = t - d.tile_base;
-        const unsigned ux0 = compact_bits16(lt) << d.tile_bits;
-        const unsigned vy0 = compact_bits16((unsigned long long)lt >> 1)
-                << d.tile_bits;
+        const unsigned ux0 = compact_bits16(lt) * (unsigned)d.tile;
+        const unsigned vy0 = compact_bits16((unsigned long long)lt >> 1) *
+                (unsigned)d.tile;
         const int origin = (int)(ux0 | (vy0 << 16));
         unsigned o = ioff[t];
         for (unsigned first = 0; first < c; first += kJobPoints, ++o)
@@ -2903,8 +2948,7 @@ __global__ void __launch_bounds__(256) k_gather_batch(
             if ((dk >> d.pos_shift) == 0) flags &= ~kNewPos;
             if ((dk & ((1ull << d.pos_shift) - 1)) == 0) flags &= ~kNewTab;
         }
-        meta[m] = make_int4(region_offset(r.pos, d.R,
-                (1 << d.tile_bits) - 1), (int)tref,
+        meta[m] = make_int4(region_offset(r.pos, d.R, d.tile), (int)tref,
                 (int)((unsigned)plane | ((unsigned)pi.srank << 16) | flags),
                 0);
         double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
@@ -3059,7 +3103,7 @@ int Batch::add(Plan* plan, size_t n, const float* uu, const float* vv,
         fail(kErrInvalidArgument, "batched jobs need GW_SINGLE plans");
     if (!grid || (n && (!uu || !vv || !ww || !vis)))
         fail(kErrInvalidArgument, "device pointers must not be NULL");
-    if (plan->key_bits() + 1 > kBatchJobShift)
+    if (plan->key_bits_max() + 1 > kBatchJobShift)
         fail(kErrInvalidArgument, "grid too large for a batched job");
     if (jobs_.size() >= 65535)
         fail(kErrInvalidArgument, "a batch holds at most 65535 jobs");
@@ -3103,6 +3147,7 @@ void Batch::prepare()
         d.grid = e.grid;
         d.G = pl.grid_size();
         d.R = pl.region();
+        d.tile = pl.tile();
         d.tile_bits = pl.tile_bits();
         d.tile_shift = pl.tile_shift();
         d.key_bits = pl.key_bits();
@@ -3111,7 +3156,7 @@ void Batch::prepare()
         d.sub_k = pl.sub_tables();
         d.pos_shift = pl.tile_shift() - 2 * d.tile_bits - pl.srank_bits();
         d.tile_base = (unsigned)tiles;
-        d.ntiles = 1u << (2 * (pl.coord_bits() - pl.tile_bits()));
+        d.ntiles = 1u << (2 * pl.tile_coord_bits());
         d.point_base = (unsigned)pts;
         d.n = (unsigned)e.n;
         combo_off_[j] = combos;
@@ -3241,7 +3286,7 @@ void Batch::enqueue()
                         e.vis, pl.grid_scale_d(), pl.w_scale_d(),
                         pl.oversample(), d.G, P, pl.device_support(),
                         pl.device_srank(), pl.srank_bits(), d.sub_h,
-                        K, d.tile_bits, pl.combo_bits(), d.key_bits,
+                        K, d.tile, d.tile_bits, pl.combo_bits(), d.key_bits,
                         d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
                         static_cast<RecF*>(d_rec_) + d.point_base, combo,
                         d_sum_ + j, (unsigned*)nullptr,
@@ -3255,7 +3300,7 @@ void Batch::enqueue()
                         e.vis, pl.grid_scale_f(), pl.w_scale_f(),
                         pl.oversample(), d.G, P, pl.device_support(),
                         pl.device_srank(), pl.srank_bits(), d.sub_h,
-                        K, d.tile_bits, pl.combo_bits(), d.key_bits,
+                        K, d.tile, d.tile_bits, pl.combo_bits(), d.key_bits,
                         d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
                         static_cast<RecF*>(d_rec_) + d.point_base, combo,
                         d_sum_ + j, (unsigned*)nullptr,

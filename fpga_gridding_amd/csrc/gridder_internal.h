@@ -245,16 +245,24 @@ public:
     size_t num_w_planes() const { return P_; }
     int max_support() const { return s_max_; }
     int timings(double* ms, int max_n);
-    // Scatter shape: tile side 2^tile_bits (-1: the largest whose LDS region
-    // fits), at most slot_cap footprint cells per lane (0: the largest the
-    // precision allows), at least min_share points per stream chunk (0: the
-    // default).  Waits for the plan's stream before changing its tables.
+    // Scatter shape: tile side 2^tile_bits (-1: the largest side, of any
+    // length up to 64, whose LDS region fits), at most slot_cap footprint
+    // cells per lane (0: the largest the precision allows), at least
+    // min_share points per stream chunk (0: the default).  Waits for the
+    // plan's stream before changing its tables.
     void configure(int tile_bits, int slot_cap, int min_share);
+    // The same with the tile side given directly (1..64; -1 as above).
+    void configure_side(int tile, int slot_cap, int min_share);
+    int tile() const { return tile_; }
+    // Bits of a coordinate inside a tile (the side is at most 2^tile_bits).
     int tile_bits() const { return tile_bits_; }
+    // Bits of a tile coordinate (tiles per axis at most 2^tile_coord_bits).
+    int tile_coord_bits() const { return tile_coord_bits_; }
     int slot_cap() const { return slot_cap_; }
     int min_share() const { return min_share_; }
     int region() const { return region_; }
     int max_tile_bits() const;
+    int max_tile() const;
     // Autotuner: runs locate + scatter on the given device points (grid
     // into scratch_grid, which is overwritten) for scatter shapes around
     // the current one -- tile side, then slots per lane, then chunk minimum,
@@ -273,6 +281,7 @@ public:
     int combo_bits() const { return combo_bits_; }
     int tile_shift() const { return tile_shift_; }
     int key_bits() const { return key_bits_; }
+    int key_bits_max() const { return key_bits_max_; }
     int oversample() const { return ov_; }
     int half_sub() const { return h_; }
     int sub_tables() const { return K_; }
@@ -320,9 +329,12 @@ private:
     int combo_bits_;     // bits of a sub-cell table index
     int plane_bits_;     // bits of a plane index
     int srank_bits_;     // bits of a support rank
-    int tile_bits_;      // log2 of the tile side T
+    int tile_;           // tile side T (1..64)
+    int tile_bits_;      // bits of a coordinate inside a tile (T <= 2^bits)
+    int tile_coord_bits_; // bits of a tile coordinate
     int tile_shift_;     // key bits below the tile index
     int key_bits_;       // valid keys are < 2^key_bits_
+    int key_bits_max_;   // key_bits_ for any tile side (sort scratch size)
     int region_;         // LDS region side T + Wmax - 1 (0: none)
     std::vector<int> support_;
     std::vector<int> srank_;
@@ -396,10 +408,11 @@ struct BatchJob
     int G, R, tile_bits, tile_shift;
     int key_bits, min_share, sub_h, sub_k;
     unsigned tile_base;       // first global tile of the job
-    unsigned ntiles;          // 4^(coord_bits - tile_bits)
+    unsigned ntiles;          // 4^tile_coord_bits
     unsigned point_base;      // first global point of the job
     unsigned n;
     int pos_shift;            // position bits start here in the job's keys
+    int tile;                 // tile side
 };
 
 // Many independent FP32 gridding jobs (each with its own plan: kernel set,

@@ -200,7 +200,9 @@ void gw_plan_records(gw_plan* plan, unsigned long long* num_records,
 
 /* Scatter shape of a plan (no counterpart in the reference; a tuning knob,
  * results do not depend on it): tile side 2^tile_log2 of footprint origins
- * per work item (-1: the largest whose FP64 LDS region fits), at most
+ * per work item (-1, the default: the largest side up to 64, a power of two
+ * or not, whose FP64 LDS region fits; the autotuner also picks such
+ * sides), at most
  * slot_cap footprint cells per lane (0: the largest: 8 for FP32-product
  * plans, 6 for GW_DOUBLE), at least min_share points per stream chunk
  * (0: 8).  Waits for the plan's stream. */
@@ -208,8 +210,10 @@ GW_EXPORT
 void gw_plan_set_tuning(gw_plan* plan, int tile_log2, int slot_cap,
         int min_share, int* status);
 
-/* values[0..4] = tile_log2, slot_cap, min_share, LDS region side (0: none),
- * largest tile_log2 whose region fits.  Returns the number written. */
+/* values[0..6] = tile_log2 (-1 when the side is not a power of two),
+ * slot_cap, min_share, LDS region side (0: none), largest tile_log2 whose
+ * region fits, tile side, largest tile side whose region fits.  Returns the
+ * number written (at most max_n). */
 GW_EXPORT int gw_plan_get_tuning(gw_plan* plan, int* values, int max_n);
 
 /* Autotuner: grids the given device points (into d_scratch_grid, sized as

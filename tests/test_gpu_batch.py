@@ -207,6 +207,10 @@ def test_autotune_keeps_results():
     try:
         g0, r0 = grid_once()
         t = pl.tuning()
+        # The default side is the largest whose region fits (here 89 - 41 =
+        # 48, not a power of two); the power-of-two sides below agree with it.
+        assert t["tile"] == t["max_tile"] == 48 and t["tile_log2"] == -1
+        assert t["max_tile_log2"] == 5 and t["region"] == 48 + 40
         for tile in range(max(0, t["max_tile_log2"] - 2),
                           t["max_tile_log2"] + 1):
             for cap in (4, 6, 8):
@@ -220,7 +224,9 @@ def test_autotune_keeps_results():
         ms = pl.autotune(wl.num_points, *d, scratch, reps=1)
         assert ms > 0.0
         tuned = pl.tuning()
-        assert 0 <= tuned["tile_log2"] <= t["max_tile_log2"]
+        assert tuned["tile"] in (48, 32, 16, 8)
+        assert tuned["tile_log2"] == (-1 if tuned["tile"] == 48 else
+                                      tuned["tile"].bit_length() - 1)
         assert tuned["slot_cap"] in (4, 6, 8)
         g2, r2 = grid_once()
         assert r2[0] == r0[0] and max_rel(g2, g0) <= 2e-6
