@@ -24,7 +24,6 @@
 #include "gridder_internal.h"
 #include "host_semantics.h"
 
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstdlib>
@@ -486,6 +485,40 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_mid(
         unsigned total;
         const unsigned ex = block_exclusive_scan(v, s_tmp, &total);
         if (i < nb) partials[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+// The whole scan in one workgroup, a tile at a time with a running carry
+// (for short inputs: at most kScanOneTiles tiles).
+constexpr unsigned kScanOneTiles = 4;
+__global__ void __launch_bounds__(kScanThreads) k_scan_one(
+        const unsigned* __restrict__ in, const unsigned* __restrict__ count,
+        unsigned* __restrict__ out, unsigned* __restrict__ total_out)
+{
+    __shared__ unsigned s_tmp[17];
+    const unsigned n = *count;
+    unsigned carry = 0;
+    for (unsigned t0 = 0; t0 < n; t0 += kScanTile)
+    {
+        const size_t base = (size_t)t0 + (size_t)threadIdx.x * kScanItems;
+        unsigned v[kScanItems];
+        unsigned sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            v[k] = base + k < n ? in[base + k] : 0u;
+            sum += v[k];
+        }
+        unsigned total;
+        unsigned run = block_exclusive_scan(sum, s_tmp, &total) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            if (base + k < n) out[base + k] = run;
+            run += v[k];
+        }
         carry += total;
     }
     if (threadIdx.x == 0) *total_out = carry;
@@ -2291,6 +2324,15 @@ void Plan::set_workspace(const Workspace& w)
     d_items_ = w.items;
 }
 
+// Scratch of the plan's key sort for n points: digit histograms and their
+// offsets (radix_hist_words(n) words each), the scan's partial sums, and
+// two words (scan length, total).
+static size_t sort_scratch_bytes(size_t n)
+{
+    const size_t hw = radix_hist_words(n);
+    return (2 * hw + hw / kScanTile + 2 + 2) * sizeof(unsigned);
+}
+
 // Carves a workspace for cap points (the layout of ensure_capacity) from
 // [base, base + bytes) with 256-byte aligned parts; with base == nullptr
 // only counts.  Returns the bytes used.
@@ -2328,12 +2370,7 @@ static size_t carve_workspace(char* base, size_t cap, size_t rec_bytes,
 size_t Plan::external_workspace_bytes(size_t n) const
 {
     const size_t cap = std::max<size_t>(n, 1024);
-    size_t sort_bytes = 0;
-    check_hip(rocprim::radix_sort_pairs(nullptr, sort_bytes,
-            (unsigned long long*)nullptr, (unsigned long long*)nullptr,
-            (unsigned*)nullptr, (unsigned*)nullptr, cap, 0u,
-            (unsigned)key_bits_max_ + 1u, stream_), kErrComputeDevices,
-            "radix sort size query");
+    const size_t sort_bytes = sort_scratch_bytes(cap);
     Workspace w{};
     return carve_workspace(nullptr, cap, prec_ == kPrecDouble ?
             sizeof(RecD) : sizeof(RecF), prec_ == kPrecDouble ?
@@ -2355,11 +2392,7 @@ void Plan::use_external_workspace(void* base, size_t bytes, size_t n)
     Workspace w{};
     w.cap = cap;
     w.items_cap = cap + cap / kJobPoints + 2;
-    check_hip(rocprim::radix_sort_pairs(nullptr, w.sort_tmp_bytes,
-            (unsigned long long*)nullptr, (unsigned long long*)nullptr,
-            (unsigned*)nullptr, (unsigned*)nullptr, cap, 0u,
-            (unsigned)key_bits_max_ + 1u, stream_), kErrComputeDevices,
-            "radix sort size query");
+    w.sort_tmp_bytes = sort_scratch_bytes(cap);
     carve_workspace(static_cast<char*>(base), cap, prec_ == kPrecDouble ?
             sizeof(RecD) : sizeof(RecF), prec_ == kPrecDouble ?
             sizeof(double) : sizeof(float), w.sort_tmp_bytes, w.items_cap,
@@ -2431,10 +2464,7 @@ void Plan::ensure_capacity(size_t n)
     d_bpos_ = dmalloc<unsigned>(cap);
     d_bstart_ = dmalloc<unsigned>(cap + 1);
     d_scan_part_ = dmalloc<unsigned>(cap / kScanTile + 2);
-    size_t bytes = 0;
-    check_hip(rocprim::radix_sort_pairs(nullptr, bytes, d_keys_[0],
-            d_keys_[1], d_idx_[0], d_idx_[1], cap, 0u, (unsigned)key_bits_max_ + 1u,
-            stream_), kErrComputeDevices, "radix sort size query");
+    const size_t bytes = sort_scratch_bytes(cap);
     d_sort_tmp_ = dmalloc<char>(bytes);
     sort_tmp_bytes_ = bytes;
     // Jobs: one per tile plus one per kJobPoints points.
@@ -2444,13 +2474,21 @@ void Plan::ensure_capacity(size_t n)
 }
 
 // Exclusive scan of *count (<= max_n) u32 values on `stream`; *total gets
-// their sum.  part holds max_n / kScanTile + 2 words.
+// their sum.  part holds max_n / kScanTile + 2 words.  Up to kScanOneTiles
+// tiles one workgroup scans them in turn (one launch instead of three).
 void scan_u32_enqueue(const unsigned* in, unsigned* out, const unsigned* count,
                       size_t max_n, unsigned* total, unsigned* part,
                       hipStream_t stream)
 {
     const unsigned nb = (unsigned)std::max<size_t>(1,
             (max_n + kScanTile - 1) / kScanTile);
+    if (nb <= kScanOneTiles)
+    {
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanThreads), 0, stream,
+                in, count, out, total);
+        check_hip(hipGetLastError(), kErrKernelLaunch, "scan launch");
+        return;
+    }
     hipLaunchKernelGGL(k_scan_up, dim3(nb), dim3(kScanThreads), 0, stream,
             in, count, part);
     hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kScanThreads), 0, stream,
@@ -2508,14 +2546,20 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     typedef typename Types<T>::Rec Rec;
     typedef typename Types<T>::T2 T2;
     typedef typename Types<T>::T4 T4;
-    if (n > 0)
-    {
-        size_t bytes = sort_tmp_bytes_;
-        check_hip(rocprim::radix_sort_pairs(d_sort_tmp_, bytes, d_keys_[0],
-                d_keys_[1], d_idx_[0], d_idx_[1], n, 0u,
-                (unsigned)key_bits_ + 1u, stream_), kErrKernelLaunch,
-                "radix sort");
-    }
+    // Stable LSD sort of the keys (one more bit than the valid keys: the
+    // skipped points' key); the result lands in pair `cur`, and the other
+    // pair is free for the stages below.
+    const size_t hw = radix_hist_words(n);
+    unsigned* const hist = static_cast<unsigned*>(d_sort_tmp_);
+    unsigned* const offs = hist + hw;
+    unsigned* const part = offs + hw;
+    unsigned* const slen = part + hw / kScanTile + 2;
+    unsigned long long* kk[2] = {d_keys_[0], d_keys_[1]};
+    unsigned* vv[2] = {d_idx_[0], d_idx_[1]};
+    const int cur = radix_sort_enqueue(n, key_bits_ + 1, kk, vv, hist, offs,
+            part, slen, slen + 1, stream_);
+    const unsigned long long* const skeys = kk[cur];
+    const unsigned* const sidx = vv[cur];
     check_hip(hipEventRecord(ev_[2], stream_), kErrKernelLaunch, "event");
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream_, (int)P_,
             K_ * K_, (const unsigned*)d_combo_, (const PlaneInfo*)d_pinfo_,
@@ -2526,20 +2570,17 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     // count points per cell): run heads into the sort's spent value buffer,
     // their scan into bpos (free until the tile scan below), the runs' keys
     // into the sort's spent key buffer.
-    unsigned* const mh = d_idx_[0];
-    unsigned long long* const mkeys = d_keys_[0];
+    unsigned* const mh = vv[cur ^ 1];
+    unsigned long long* const mkeys = kk[cur ^ 1];
     hipLaunchKernelGGL(k_merge_heads, dim3(blocks), dim3(256), 0, stream_,
-            (const Summary*)d_summary_, (const unsigned long long*)d_keys_[1],
-            mh, diag ? 0 : 1);
+            (const Summary*)d_summary_, skeys, mh, diag ? 0 : 1);
     scan_u32(mh, d_bpos_, &d_summary_->nv32, n, &d_summary_->nm32);
     // Run starts into the bucket table (free until the tile scan below).
     hipLaunchKernelGGL(k_run_starts, dim3(blocks), dim3(256), 0, stream_,
             (const Summary*)d_summary_, (const unsigned*)mh,
             (const unsigned*)d_bpos_, d_bstart_);
     hipLaunchKernelGGL((k_gather<T>), dim3(blocks), dim3(256), 0, stream_,
-            d_summary_,
-            (const unsigned long long*)d_keys_[1], (const unsigned*)d_idx_[1],
-            (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
+            d_summary_, skeys, sidx, (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
             (const PlaneInfo*)d_pinfo_, h_, K_, tile_shift_, d_meta_,
             (T4*)d_vq_, d_head_, diag ? 0 : region_, tile_,
             combo_bits_ + plane_bits_, (const unsigned*)d_bstart_, mkeys);
@@ -2683,9 +2724,11 @@ constexpr unsigned long long kBatchLocalMask = (1ull << kBatchJobShift) - 1;
 // (no memset, no host round trip), so the sequence can be captured.
 // ------------------------------------------------------------------------
 
-constexpr int kRadixThreads = 256;
-constexpr int kRadixRounds = 16;
-constexpr int kRadixChunk = kRadixThreads * kRadixRounds;
+constexpr int kRadixThreads = 512;
+constexpr int kRadixKeys = 8;                   // keys per lane
+constexpr int kRadixWaves = kRadixThreads / 64;
+constexpr int kRadixChunk = kRadixThreads * kRadixKeys;
+static_assert(kRadixChunk == 4096, "chunk of the histogram layout");
 
 __global__ void __launch_bounds__(256) k_batch_key(size_t n,
         const unsigned long long* __restrict__ keys,
@@ -2714,51 +2757,36 @@ __global__ void __launch_bounds__(256) k_batch_key(size_t n,
 }
 
 // Digit histogram of each chunk of kRadixChunk keys, digit-major:
-// hist[digit * nblocks + block].
+// hist[digit * nblocks + block].  Wave w counts keys [w * 512, w * 512 +
+// 512) of the chunk into its own LDS histogram (all its loads first); lanes
+// with equal digits are found by ballots and counted by one add, since
+// neighbouring keys often share a digit (the high digits of sorted keys).
 __global__ void __launch_bounds__(kRadixThreads) k_radix_hist(size_t n,
         const unsigned long long* __restrict__ keys, int shift,
         unsigned nblocks, unsigned* __restrict__ hist)
 {
-    __shared__ unsigned h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const size_t b0 = (size_t)blockIdx.x * kRadixChunk + threadIdx.x;
-#pragma unroll 4
-    for (int r = 0; r < kRadixRounds; ++r)
-    {
-        const size_t i = b0 + (size_t)r * kRadixThreads;
-        if (i < n) atomicAdd(&h[(unsigned)(keys[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
-}
-
-// Stable scatter of each chunk by digit: offs is the exclusive scan of
-// hist; inside a chunk, keys keep their order (rounds in order, waves in
-// order inside a round, lanes in order inside a wave via ballot matching).
-__global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
-        const unsigned long long* __restrict__ keys,
-        const unsigned* __restrict__ vals, int shift, unsigned nblocks,
-        const unsigned* __restrict__ offs,
-        unsigned long long* __restrict__ keys_out,
-        unsigned* __restrict__ vals_out)
-{
-    constexpr int kWaves = kRadixThreads / 64;
-    __shared__ unsigned base[256];
-    __shared__ unsigned cnt[kWaves][256];
+    __shared__ unsigned h[kRadixWaves][256];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
-    const unsigned long long below = (1ull << lane) - 1;
-    base[tid] = offs[(size_t)tid * nblocks + blockIdx.x];
-    const size_t b0 = (size_t)blockIdx.x * kRadixChunk + tid;
-    for (int r = 0; r < kRadixRounds; ++r)
+    for (int i = tid; i < kRadixWaves * 256; i += kRadixThreads)
+        (&h[0][0])[i] = 0;
+    const size_t base = (size_t)blockIdx.x * kRadixChunk +
+            (size_t)w * (64 * kRadixKeys) + lane;
+    unsigned long long k[kRadixKeys];
+#pragma unroll
+    for (int j = 0; j < kRadixKeys; ++j)
     {
-        const size_t i = b0 + (size_t)r * kRadixThreads;
-        const bool ok = i < n;
-        const unsigned long long k = ok ? keys[i] : 0ull;
-        const unsigned v = ok ? vals[i] : 0u;
-        const unsigned d = (unsigned)(k >> shift) & 255u;
+        const size_t i = base + (size_t)j * 64;
+        k[j] = i < n ? keys[i] : 0ull;
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int j = 0; j < kRadixKeys; ++j)
+    {
+        const bool ok = base + (size_t)j * 64 < n;
+        const unsigned d = (unsigned)(k[j] >> shift) & 255u;
         unsigned long long peers = __ballot(ok);
 #pragma unroll
         for (int b = 0; b < 8; ++b)
@@ -2766,23 +2794,128 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
             const unsigned long long m = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? m : ~m;
         }
+        if (ok && (peers & below) == 0)
+            h[w][d] += (unsigned)__popcll(peers);
+    }
+    __syncthreads();
+    if (tid < 256)
+    {
+        unsigned c = 0;
 #pragma unroll
-        for (int q = 0; q < kWaves; ++q) cnt[q][tid] = 0;
-        __syncthreads();
-        if (ok && (peers & below) == 0) cnt[w][d] = __popcll(peers);
-        __syncthreads();
+        for (int q = 0; q < kRadixWaves; ++q) c += h[q][tid];
+        hist[(size_t)tid * nblocks + blockIdx.x] = c;
+    }
+}
+
+// Stable scatter of each chunk by digit; offs is the exclusive scan of
+// hist.  Wave w ranks keys [w * 512, w * 512 + 512) of the chunk in order
+// (lanes with equal digits found by ballots, a running count per digit in
+// the wave's own LDS row), the waves' counts give each wave's start per
+// digit inside the chunk, the chunk is staged in LDS in digit order, and
+// each digit's run is written out contiguously at its global offset.
+__global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
+        const unsigned long long* __restrict__ keys,
+        const unsigned* __restrict__ vals, int shift, unsigned nblocks,
+        const unsigned* __restrict__ offs,
+        unsigned long long* __restrict__ keys_out,
+        unsigned* __restrict__ vals_out)
+{
+    __shared__ unsigned long long sk[kRadixChunk];
+    __shared__ unsigned sv[kRadixChunk];
+    __shared__ unsigned wh[kRadixWaves][256];
+    __shared__ unsigned gof[256];
+    __shared__ unsigned wsum[4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const size_t c0 = (size_t)blockIdx.x * kRadixChunk;
+    const size_t base = c0 + (size_t)w * (64 * kRadixKeys) + lane;
+    for (int i = tid; i < kRadixWaves * 256; i += kRadixThreads)
+        (&wh[0][0])[i] = 0;
+    unsigned long long k[kRadixKeys];
+    unsigned v[kRadixKeys], dg[kRadixKeys], rk[kRadixKeys];
+#pragma unroll
+    for (int j = 0; j < kRadixKeys; ++j)
+    {
+        const size_t i = base + (size_t)j * 64;
+        k[j] = i < n ? keys[i] : 0ull;
+        v[j] = i < n ? vals[i] : 0u;
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int j = 0; j < kRadixKeys; ++j)
+    {
+        const bool ok = base + (size_t)j * 64 < n;
+        const unsigned d = (unsigned)(k[j] >> shift) & 255u;
+        unsigned long long peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+        {
+            const unsigned long long m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        dg[j] = d;
+        rk[j] = 0;
         if (ok)
         {
-            unsigned pos = base[d] + __popcll(peers & below);
-            for (int q = 0; q < w; ++q) pos += cnt[q][d];
-            keys_out[pos] = k;
-            vals_out[pos] = v;
+            // This wave's keys of digit d before this one: the running
+            // count, then the lanes below with the same digit; the lowest
+            // such lane advances the count.
+            const unsigned pre = wh[w][d];
+            rk[j] = pre + (unsigned)__popcll(peers & below);
+            if ((peers & below) == 0)
+                wh[w][d] = pre + (unsigned)__popcll(peers);
         }
-        __syncthreads();
-        unsigned add = 0;
+    }
+    __syncthreads();
+    // Digit tid: the waves' starts (exclusive over waves), the chunk's
+    // start of the digit (exclusive over digits) and its global offset.
+    unsigned acc = 0, inc = 0;
+    if (tid < 256)
+    {
 #pragma unroll
-        for (int q = 0; q < kWaves; ++q) add += cnt[q][tid];
-        base[tid] += add;
+        for (int q = 0; q < kRadixWaves; ++q)
+        {
+            const unsigned c = wh[q][tid];
+            wh[q][tid] = acc;
+            acc += c;
+        }
+        inc = acc;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const unsigned y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) wsum[w] = inc;
+    }
+    __syncthreads();
+    if (tid < 256)
+    {
+        unsigned ls = inc - acc;
+        for (int q = 0; q < w; ++q) ls += wsum[q];
+#pragma unroll
+        for (int q = 0; q < kRadixWaves; ++q) wh[q][tid] += ls;
+        gof[tid] = offs[(size_t)tid * nblocks + blockIdx.x] - ls;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRadixKeys; ++j)
+        if (base + (size_t)j * 64 < n)
+        {
+            const unsigned p = wh[w][dg[j]] + rk[j];
+            sk[p] = k[j];
+            sv[p] = v[j];
+        }
+    __syncthreads();
+    const unsigned cnt = (unsigned)min((size_t)kRadixChunk, n - c0);
+    for (unsigned p = tid; p < cnt; p += kRadixThreads)
+    {
+        const unsigned long long kk = sk[p];
+        const unsigned g = gof[(unsigned)(kk >> shift) & 255u] + p;
+        keys_out[g] = kk;
+        vals_out[g] = sv[p];
     }
 }
 
