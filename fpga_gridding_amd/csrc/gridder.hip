@@ -1422,10 +1422,10 @@ __device__ __forceinline__ void flush_region_comp(GridLds& L, int R, int y0,
 // plane segments found with a block scan, streams of waves per segment, and
 // with an LDS region (lds) the region added to the grid at the end and
 // cleared, sink cells included (the next item may have another R).
-template <typename T, bool DIAG, typename TapT, bool COMP>
+template <typename T, bool DIAG, typename TapT, bool COMP, typename Pre>
 __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
         const PlaneInfo* __restrict__ pinfo, const int4& job, bool lds,
-        int min_share)
+        int min_share, Pre&& pre)
 {
     const int tid = threadIdx.x;
     const int NT = blockDim.x;
@@ -1507,6 +1507,11 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
                         sb, cb);
         }
     }
+    // The next work item is requested as this one's points are done, so
+    // that the queue round trip overlaps the region flush (see
+    // for_each_item); earlier, more tiles would be in flight per XCD and
+    // their tap tables would share its L2.
+    pre();
     __syncthreads();
     if (lds)
     {
@@ -1541,16 +1546,20 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
     }
 }
 
-// Calls f(it) for every scatter item it of [0, s->total_items), each item
-// in exactly one workgroup (persistent grid).  xcds <= 1 (or more than the
-// queues in Summary): grid-stride order.
+// Calls f(it, pre) for every scatter item it of [0, s->total_items), each
+// item in exactly one workgroup (persistent grid).  xcds <= 1 (or more than
+// the queues in Summary): grid-stride order.
 // Otherwise: items are in tile Z-order, so a contiguous range of them covers
 // one compact grid region, whose points use a narrower set of w-planes than
 // the whole grid.  Workgroups are dispatched to the XCDs round robin
 // (workgroup b runs on XCD b mod xcds); XCD x takes the x-th of xcds
 // contiguous ranges from its own queue (s->item_next[x], zero at launch), so
 // its L2 holds the tap tables of that region only, then helps the other
-// queues in turn.  Every workgroup must call this (it synchronises).
+// queues in turn.  f calls pre() once, after its first barrier and before
+// its last one: lane 0 of the last wave then requests the next index of the
+// same queue into *s_next, and the index is read after f returns, so the
+// atomic's round trip overlaps the end of the item instead of preceding the
+// next one.  Every workgroup must call this (it synchronises).
 // f is called from one place only, so that it is inlined once: an
 // out-of-line f would receive the LDS region through a generic pointer and
 // its atomics would become flat atomics.
@@ -1563,38 +1572,53 @@ __device__ __forceinline__ void for_each_item(Summary* __restrict__ s,
     const unsigned nq = queues ? (unsigned)xcds : 1u;
     const unsigned per = (total + nq - 1) / nq;
     const unsigned home = blockIdx.x % nq;
-    unsigned next = blockIdx.x;     // grid-stride order
+    const bool leader = (int)threadIdx.x == (int)blockDim.x - 64;
     unsigned q = 0;                 // queues: the q-th after the home queue
-    for (;;)
-    {
-        unsigned it = ~0u;
-        if (!queues)
+    // The next item of queue q on, synchronously (~0u once all are empty).
+    auto take = [&]() -> unsigned {
+        for (; q < nq; ++q)
         {
-            it = next < total ? next : ~0u;
-            next += gridDim.x;
+            const unsigned xq = (home + q) % nq;
+            const unsigned lo = xq * per;
+            const unsigned hi = min(total, lo + per);
+            if (threadIdx.x == 0)
+                *s_next = lo + atomicAdd(&s->item_next[xq], 1u);
+            __syncthreads();
+            const unsigned got = (unsigned)__builtin_amdgcn_readfirstlane(
+                    (int)*s_next);
+            __syncthreads();
+            if (got < hi) return got;
         }
+        return ~0u;
+    };
+    if (!queues)
+    {
+        for (unsigned it = blockIdx.x; it < total; it += gridDim.x)
+            f(it, [] {});
+        return;
+    }
+    unsigned it = take();
+    while (it != ~0u)
+    {
+        const unsigned xq = (home + q) % nq;
+        const unsigned lo = xq * per;
+        const unsigned hi = min(total, lo + per);
+        f(it, [&] {
+            if (leader) *s_next = lo + atomicAdd(&s->item_next[xq], 1u);
+        });
+        // f's last barrier orders the store above before this read, and
+        // f's first barrier orders this read before the next store.
+        const unsigned got = (unsigned)__builtin_amdgcn_readfirstlane(
+                (int)*s_next);
+        if (got < hi)
+            it = got;
         else
         {
-            for (; q < nq; ++q)
-            {
-                const unsigned xq = (home + q) % nq;
-                const unsigned lo = xq * per;
-                const unsigned hi = min(total, lo + per);
-                if (threadIdx.x == 0)
-                    *s_next = lo + atomicAdd(&s->item_next[xq], 1u);
-                __syncthreads();
-                const unsigned got = (unsigned)__builtin_amdgcn_readfirstlane(
-                        (int)*s_next);
-                __syncthreads();
-                if (got < hi)
-                {
-                    it = got;
-                    break;
-                }
-            }
+            // Every wave has read *s_next before take() writes it.
+            __syncthreads();
+            ++q;
+            it = take();
         }
-        if (it == ~0u) break;
-        f(it);
     }
 }
 
@@ -1634,9 +1658,9 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
         L.re[i] = 0.0;
         L.im[i] = 0.0;
     }
-    for_each_item(summary, xcds, &s_next, [&](unsigned it) {
+    for_each_item(summary, xcds, &s_next, [&](unsigned it, auto&& pre) {
         grid_work_item<T, DIAG, TapT, COMP>(L, x, pinfo, items[it], lds,
-                                            min_share);
+                                            min_share, pre);
     });
 }
 
@@ -3132,7 +3156,7 @@ __global__ void __launch_bounds__(kGridThreads) k_grid_batch(
     x.diag_count = nullptr;
     x.diag_wsum = nullptr;
     x.row0 = 0;
-    for_each_item(bsum, xcds, &s_next, [&](unsigned it) {
+    for_each_item(bsum, xcds, &s_next, [&](unsigned it, auto&& pre) {
         const int4 item = items[it];
         // BatchJob fields by word: pinfo, etab, grid (2 words each), G, R,
         // tile_bits, tile_shift, key_bits, min_share, ...
@@ -3147,7 +3171,7 @@ __global__ void __launch_bounds__(kGridThreads) k_grid_batch(
         x.R = (int)w[7];
         const int min_share = (int)w[11];
         grid_work_item<float, false, float2, false>(L, x, pinfo, item,
-                x.R > 0, min_share);
+                x.R > 0, min_share, pre);
     });
 }
 
