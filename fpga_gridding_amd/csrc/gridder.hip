@@ -2349,12 +2349,10 @@ void Plan::set_workspace(const Workspace& w)
 }
 
 // Scratch of the plan's key sort for n points: digit histograms and their
-// offsets (radix_hist_words(n) words each), the scan's partial sums, and
-// two words (scan length, total).
+// offsets (radix_hist_words(n) words each) and the 256 digit totals.
 static size_t sort_scratch_bytes(size_t n)
 {
-    const size_t hw = radix_hist_words(n);
-    return (2 * hw + hw / kScanTile + 2 + 2) * sizeof(unsigned);
+    return (2 * radix_hist_words(n) + 256) * sizeof(unsigned);
 }
 
 // Carves a workspace for cap points (the layout of ensure_capacity) from
@@ -2576,12 +2574,10 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     const size_t hw = radix_hist_words(n);
     unsigned* const hist = static_cast<unsigned*>(d_sort_tmp_);
     unsigned* const offs = hist + hw;
-    unsigned* const part = offs + hw;
-    unsigned* const slen = part + hw / kScanTile + 2;
     unsigned long long* kk[2] = {d_keys_[0], d_keys_[1]};
     unsigned* vv[2] = {d_idx_[0], d_idx_[1]};
     const int cur = radix_sort_enqueue(n, key_bits_ + 1, kk, vv, hist, offs,
-            part, slen, slen + 1, stream_);
+            offs + hw, stream_);
     const unsigned long long* const skeys = kk[cur];
     const unsigned* const sidx = vv[cur];
     check_hip(hipEventRecord(ev_[2], stream_), kErrKernelLaunch, "event");
@@ -2831,8 +2827,10 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_hist(size_t n,
     }
 }
 
-// Stable scatter of each chunk by digit; offs is the exclusive scan of
-// hist.  Wave w ranks keys [w * 512, w * 512 + 512) of the chunk in order
+// Stable scatter of each chunk by digit; offs[d * nblocks + b] is the
+// exclusive scan of digit d's chunk counts over the chunks and tot[d] the
+// digit's total, so the digit's keys start at the exclusive scan of tot.
+// Wave w ranks keys [w * 512, w * 512 + 512) of the chunk in order
 // (lanes with equal digits found by ballots, a running count per digit in
 // the wave's own LDS row), the waves' counts give each wave's start per
 // digit inside the chunk, the chunk is staged in LDS in digit order, and
@@ -2840,7 +2838,7 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_hist(size_t n,
 __global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
         const unsigned long long* __restrict__ keys,
         const unsigned* __restrict__ vals, int shift, unsigned nblocks,
-        const unsigned* __restrict__ offs,
+        const unsigned* __restrict__ offs, const unsigned* __restrict__ tot,
         unsigned long long* __restrict__ keys_out,
         unsigned* __restrict__ vals_out)
 {
@@ -2849,6 +2847,7 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
     __shared__ unsigned wh[kRadixWaves][256];
     __shared__ unsigned gof[256];
     __shared__ unsigned wsum[4];
+    __shared__ unsigned tsum[4];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
@@ -2894,8 +2893,9 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
     }
     __syncthreads();
     // Digit tid: the waves' starts (exclusive over waves), the chunk's
-    // start of the digit (exclusive over digits) and its global offset.
-    unsigned acc = 0, inc = 0;
+    // start of the digit (exclusive over digits), and the digit's start in
+    // the output (exclusive scan of the totals) plus this chunk's offset.
+    unsigned acc = 0, inc = 0, t = 0, tinc = 0;
     if (tid < 256)
     {
 #pragma unroll
@@ -2906,22 +2906,37 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
             acc += c;
         }
         inc = acc;
+        t = tot[tid];
+        tinc = t;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1)
         {
             const unsigned y = __shfl_up(inc, o);
-            if (lane >= o) inc += y;
+            const unsigned z = __shfl_up(tinc, o);
+            if (lane >= o)
+            {
+                inc += y;
+                tinc += z;
+            }
         }
-        if (lane == 63) wsum[w] = inc;
+        if (lane == 63)
+        {
+            wsum[w] = inc;
+            tsum[w] = tinc;
+        }
     }
     __syncthreads();
     if (tid < 256)
     {
-        unsigned ls = inc - acc;
-        for (int q = 0; q < w; ++q) ls += wsum[q];
+        unsigned ls = inc - acc, ds = tinc - t;
+        for (int q = 0; q < w; ++q)
+        {
+            ls += wsum[q];
+            ds += tsum[q];
+        }
 #pragma unroll
         for (int q = 0; q < kRadixWaves; ++q) wh[q][tid] += ls;
-        gof[tid] = offs[(size_t)tid * nblocks + blockIdx.x] - ls;
+        gof[tid] = ds + offs[(size_t)tid * nblocks + blockIdx.x] - ls;
     }
     __syncthreads();
 #pragma unroll
@@ -2943,38 +2958,65 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
     }
 }
 
-__global__ void k_set_u32(unsigned* p, unsigned v)
-{
-    *p = v;
-}
-
 size_t radix_hist_words(size_t n)
 {
     return 256 * ((n + kRadixChunk - 1) / kRadixChunk);
 }
 
+// Digit d per workgroup: exclusive scan of its chunk counts hist[d * nb +
+// b] over the chunks b into offs, its total into tot[d].
+__global__ void __launch_bounds__(kScanThreads) k_radix_scan(
+        const unsigned* __restrict__ hist, unsigned nb,
+        unsigned* __restrict__ offs, unsigned* __restrict__ tot)
+{
+    __shared__ unsigned s_tmp[17];
+    const unsigned* const h = hist + (size_t)blockIdx.x * nb;
+    unsigned* const o = offs + (size_t)blockIdx.x * nb;
+    unsigned carry = 0;
+    for (unsigned t0 = 0; t0 < nb; t0 += kScanTile)
+    {
+        const unsigned base = t0 + threadIdx.x * kScanItems;
+        unsigned v[kScanItems];
+        unsigned sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            v[k] = base + k < nb ? h[base + k] : 0u;
+            sum += v[k];
+        }
+        unsigned total;
+        unsigned run = block_exclusive_scan(sum, s_tmp, &total) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            if (base + k < nb) o[base + k] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
 int radix_sort_enqueue(size_t n, int bits, unsigned long long* keys[2],
                        unsigned* vals[2], unsigned* hist, unsigned* offs,
-                       unsigned* part, unsigned* d_len, unsigned* d_total,
-                       hipStream_t s)
+                       unsigned* tot, hipStream_t s)
 {
     const unsigned nchunk = (unsigned)((n + kRadixChunk - 1) / kRadixChunk);
     if (nchunk == 0) return 0;
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, d_len,
-            256u * nchunk);
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8)
     {
         hipLaunchKernelGGL(k_radix_hist, dim3(nchunk), dim3(kRadixThreads),
                 0, s, n, (const unsigned long long*)keys[cur], shift, nchunk,
                 hist);
-        scan_u32_enqueue(hist, offs, d_len, 256 * (size_t)nchunk, d_total,
-                part, s);
+        hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(kScanThreads), 0, s,
+                (const unsigned*)hist, nchunk, offs, tot);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nchunk),
                 dim3(kRadixThreads), 0, s, n,
                 (const unsigned long long*)keys[cur],
                 (const unsigned*)vals[cur], shift, nchunk,
-                (const unsigned*)offs, keys[cur ^ 1], vals[cur ^ 1]);
+                (const unsigned*)offs, (const unsigned*)tot, keys[cur ^ 1],
+                vals[cur ^ 1]);
         cur ^= 1;
     }
     check_hip(hipGetLastError(), kErrKernelLaunch, "radix sort launch");
@@ -3193,7 +3235,7 @@ Batch::Batch()
     for (auto& e : ev_)
         check_hip(hipEventCreate(&e), kErrComputeDevices, "hipEventCreate");
     d_bsum_ = dmalloc<Summary>(1);
-    d_ntiles_ = dmalloc<unsigned>(2);
+    d_ntiles_ = dmalloc<unsigned>(1);
     check_hip(hipHostMalloc((void**)&h_ntiles_, sizeof(unsigned),
             hipHostMallocDefault), kErrMemAlloc, "hipHostMalloc");
     check_hip(hipEventCreateWithFlags(&ev_desc_, hipEventDisableTiming),
@@ -3371,8 +3413,9 @@ void Batch::prepare()
         d_roff_ = dmalloc<unsigned>(rh);
         d_icnt_ = dmalloc<unsigned>(cap_tiles_);
         d_ioff_ = dmalloc<unsigned>(cap_tiles_);
-        d_part_ = dmalloc<unsigned>(
-                std::max(cap_points_, cap_tiles_) / kScanTile + 2);
+        // Scan partial sums; also the sort's 256 digit totals.
+        d_part_ = dmalloc<unsigned>(std::max<size_t>(256,
+                std::max(cap_points_, cap_tiles_) / kScanTile + 2));
         d_items_ = dmalloc<int4>(cap_points_ + cap_points_ / kJobPoints + 2);
         d_desc_ = dmalloc<BatchJob>(cap_jobs_);
         d_sum_ = dmalloc<Summary>(cap_jobs_);
@@ -3486,7 +3529,7 @@ void Batch::enqueue()
     unsigned long long* kk[2] = {d_keys_[1], d_keys_[0]};
     unsigned* vv[2] = {d_idx_[0], d_idx_[1]};
     const int cur = radix_sort_enqueue(n_total_, radix_bits_, kk, vv,
-            d_rhist_, d_roff_, d_part_, d_ntiles_ + 1, &d_bsum_->n_buckets, s);
+            d_rhist_, d_roff_, d_part_, s);
     const unsigned long long* kin = kk[cur];
     const unsigned* vin = vv[cur];
     // Runs of equal keys (same job, tile, origin, plane and sub-cell table)

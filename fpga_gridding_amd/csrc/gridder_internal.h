@@ -497,7 +497,7 @@ private:
     unsigned* d_part_ = nullptr;
     unsigned* d_rhist_ = nullptr;     // radix digit counts per chunk
     unsigned* d_roff_ = nullptr;      // their exclusive scan
-    unsigned* d_ntiles_ = nullptr;    // [tiles, radix histogram length]
+    unsigned* d_ntiles_ = nullptr;    // tiles (the tile scan's length)
     // Pinned staging of the job table and tile count, copied on the
     // batch's stream (stream order keeps the previous run's reads first);
     // ev_desc_ marks the copy done before the staging is reused.
@@ -520,16 +520,14 @@ private:
 };
 
 // Stable LSD radix sort of n (key, value) pairs on the low `bits` bits of
-// the keys (8-bit digits, one pass per digit), capture-safe.  keys[0] /
-// vals[0] hold the input, keys[1] / vals[1] are scratch of the same size;
-// returns which pair holds the result.  hist and offs hold
-// radix_hist_words(n) words, part the scan's partial sums
-// (radix_hist_words(n) / 4096 + 2 words), d_len and d_total one word each.
+// the keys (8-bit digits, one pass per digit: histogram, per-digit scan,
+// scatter), capture-safe.  keys[0] / vals[0] hold the input, keys[1] /
+// vals[1] are scratch of the same size; returns which pair holds the
+// result.  hist and offs hold radix_hist_words(n) words, tot 256 words.
 size_t radix_hist_words(size_t n);
 int radix_sort_enqueue(size_t n, int bits, unsigned long long* keys[2],
                        unsigned* vals[2], unsigned* hist, unsigned* offs,
-                       unsigned* part, unsigned* d_len, unsigned* d_total,
-                       hipStream_t s);
+                       unsigned* tot, hipStream_t s);
 
 // Uniform-weighting step (oskar_grid_weights_write / _read) on device
 // pointers, FP32 (kPrecSingle) or FP64 (kPrecDouble); each call runs on `s`
@@ -570,7 +568,7 @@ private:
     unsigned* d_hist_ = nullptr;
     unsigned* d_offs_ = nullptr;
     unsigned* d_part_ = nullptr;
-    unsigned long long* d_small_ = nullptr;   // skipped; scan length, total
+    unsigned long long* d_small_ = nullptr;   // skipped points
     unsigned long long* h_skipped_ = nullptr;
 };
 

@@ -131,7 +131,7 @@ Weights::Weights()
     check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
             kErrComputeDevices, "hipStreamCreateWithFlags");
     own_stream_ = true;
-    d_small_ = dmalloc<unsigned long long>(2);
+    d_small_ = dmalloc<unsigned long long>(1);
     check_hip(hipHostMalloc((void**)&h_skipped_, sizeof(unsigned long long),
             hipHostMallocDefault), kErrMemAlloc, "hipHostMalloc");
 }
@@ -184,7 +184,7 @@ void Weights::ensure_capacity(size_t n)
     const size_t hw = radix_hist_words(cap);
     d_hist_ = dmalloc<unsigned>(hw);
     d_offs_ = dmalloc<unsigned>(hw);
-    d_part_ = dmalloc<unsigned>(hw / 4096 + 2);
+    d_part_ = dmalloc<unsigned>(256);        // the sort's digit totals
     cap_ = cap;
 }
 
@@ -192,7 +192,7 @@ size_t Weights::device_bytes() const
 {
     const size_t hw = cap_ ? radix_hist_words(cap_) : 0;
     return cap_ * 2 * (sizeof(unsigned long long) + sizeof(unsigned)) +
-           (2 * hw + hw / 4096 + 2) * sizeof(unsigned);
+           (2 * hw + 256) * sizeof(unsigned);
 }
 
 static void check_weights_args(size_t n, const void* uu, const void* vv,
@@ -232,8 +232,7 @@ size_t Weights::write_t(size_t n, const T* uu, const T* vv, const T* weight,
             st, n, uu, vv, scale, G, cells, d_keys_[0], d_idx_[0],
             d_small_);
     const int cur = radix_sort_enqueue(n, bits, d_keys_, d_idx_, d_hist_,
-            d_offs_, d_part_, reinterpret_cast<unsigned*>(d_small_ + 1),
-            reinterpret_cast<unsigned*>(d_small_ + 1) + 1, st);
+            d_offs_, d_part_, st);
     hipLaunchKernelGGL(k_weights_sum<T>, dim3(blocks), dim3(kThreads), 0,
             st, n, (const unsigned long long*)d_keys_[cur],
             (const unsigned*)d_idx_[cur], weight, cells, grid);
