@@ -115,6 +115,51 @@ def test_shapes_fp32(case):
     assert_fp32_parity(r)
 
 
+@pytest.mark.parametrize("n", [1, 4095, 4096, 4097, 8193])
+def test_sort_chunk_boundaries(n):
+    """Point counts around the key sort's 4096-key chunks (a lone key, a
+    chunk short of full, exactly full, one over, two chunks and one key)."""
+    wl = synth.random_small(n=n, grid_size=128, num_planes=6, s_min=2,
+                            s_max=6, seed=20 + n)
+    g, sk, nrm = fg.grid_workload_f(wl)
+    r = fp32_report(g, wl)
+    assert sk == r["skipped_f"]
+    assert abs(nrm - r["norm_f"]) <= 1e-12 * max(1.0, abs(r["norm_f"]))
+    assert_fp32_parity(r)
+
+
+def test_all_points_one_cell():
+    """9000 points with the same uvw: every sort key and every digit is the
+    same (one run, each chunk's keys in one digit), then two cells taking
+    turns (two runs, interleaved input)."""
+    base = synth.random_small(n=9000, grid_size=128, num_planes=6, s_min=2,
+                              s_max=6, seed=31, edge_fraction=0.0)
+    kept = np.flatnonzero(oracle.locate_f(base)[0]["skip"] == 0)[:2]
+    for cells in (1, 2):
+        pick = kept[np.arange(base.num_points) % cells]
+        wl = dataclasses.replace(
+            base, uu=base.uu[pick].copy(), vv=base.vv[pick].copy(),
+            ww=base.ww[pick].copy())
+        g, sk, nrm = fg.grid_workload_f(wl)
+        r = fp32_report(g, wl)
+        assert sk == r["skipped_f"] == 0
+        assert abs(nrm - r["norm_f"]) <= 1e-12 * abs(r["norm_f"])
+        assert_fp32_parity(r)
+        pl = fg.Plan.for_workload(wl)
+        try:
+            d = [fg.DeviceArray.from_host(a)
+                 for a in (wl.uu, wl.vv, wl.ww, wl.vis)]
+            gd = fg.DeviceArray(8 * wl.grid_size ** 2)
+            gd.zero()
+            pl.grid(wl.num_points, *d, gd)
+            pl.results()
+            assert pl.records()[0] == cells
+            for x in (*d, gd):
+                x.close()
+        finally:
+            pl.close()
+
+
 @pytest.mark.parametrize("case", SHAPES[:4], ids=lambda c: f"ov{c['ov']}")
 def test_shapes_fp64(case):
     wl = synth.random_small(**case)
