@@ -801,6 +801,49 @@ __device__ __forceinline__ unsigned compact_bits16(unsigned long long x)
     return (unsigned)x;
 }
 
+// Support segments of every work item, one wave per item: a segment starts
+// at the item's first record and wherever the support rank (meta.z >> 16)
+// changes (records are sorted by rank within a tile).  Segment g of item it
+// (first record f) goes to segs[f + g] as its first record in the item |
+// its plane << 12, and the count to bits 16-31 of items[it].z (which holds
+// the item's records, <= kJobPoints, below).  The scatter then starts an
+// item with a few scalar loads instead of a block scan.
+__global__ void __launch_bounds__(256) k_item_segments(
+        const Summary* __restrict__ summary, unsigned items_cap,
+        int4* __restrict__ items, const int4* __restrict__ meta,
+        unsigned* __restrict__ segs)
+{
+    const unsigned n = min(summary->total_items, items_cap);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned waves = gridDim.x * (blockDim.x >> 6);
+    for (unsigned it = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+         it < n; it += waves)
+    {
+        const int4 job = items[it];
+        const int f = job.y, len = job.z & 0xFFFF;
+        unsigned cnt = 0;
+        for (int b = 0; b < len; b += 64)
+        {
+            const int i = b + lane;
+            bool head = false;
+            unsigned z = 0;
+            if (i < len)
+            {
+                z = (unsigned)meta[f + i].z;
+                head = i == 0 || (((unsigned)meta[f + i - 1].z >> 16) &
+                        kRankMask) != ((z >> 16) & kRankMask);
+            }
+            const unsigned long long m = __ballot(head);
+            if (head)
+                segs[f + cnt + (unsigned)__popcll(m & below)] =
+                        (unsigned)i | ((z & 0xFFFFu) << 12);
+            cnt += (unsigned)__popcll(m);
+        }
+        if (lane == 0) items[it].z = len | (int)(cnt << 16);
+    }
+}
+
 // Job = (tile origin u | v << 16, first sorted point, points, 0).  Also
 // resets the scatter's queue heads (the kernel runs right before it).
 __global__ void __launch_bounds__(256) k_write_items(
@@ -861,10 +904,6 @@ struct GridLds
 {
     double re[kRegionCells];      // region real parts, then the sink cells
     double im[kRegionCells];      // region imaginary parts, then the sinks
-    unsigned seg[kJobPoints + 1]; // plane segment starts within the job
-    int segp[kJobPoints];         // plane of each segment
-    unsigned tmp[17];
-    unsigned nseg;
 };
 constexpr unsigned kImBytes = kRegionCells * sizeof(double);
 static_assert(kImBytes < 65536, "imaginary offset must fit a DS offset");
@@ -924,6 +963,7 @@ struct ScatterCtx
     const int4* __restrict__ meta;
     const T4* __restrict__ vq;
     const void* __restrict__ etab;   // tap pairs of type TapT
+    const unsigned* __restrict__ segs; // per work item: its segments
     GridLds* lds;
     T* __restrict__ grid;
     unsigned* __restrict__ diag_count;
@@ -1418,8 +1458,9 @@ __device__ __forceinline__ void flush_region_comp(GridLds& L, int R, int y0,
 }
 
 // One work item (up to kJobPoints sorted points of one tile, origin
-// job.x = u | v << 16, points [job.y, job.y + job.z)) by one workgroup:
-// plane segments found with a block scan, streams of waves per segment, and
+// job.x = u | v << 16, points [job.y, job.y + (job.z & 0xFFFF))) by one
+// workgroup: its support segments as k_item_segments listed them, streams
+// of waves per segment, and
 // with an LDS region (lds) the region added to the grid at the end and
 // cleared, sink cells included (the next item may have another R).
 template <typename T, bool DIAG, typename TapT, bool COMP, typename Pre>
@@ -1432,59 +1473,23 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int R = x.R, G = x.G, row0 = x.row0;
     T* const grid = x.grid;
-    const int4* __restrict__ meta = x.meta;
     x.ux0 = job.x & 0xFFFF;
     x.vy0 = (int)((unsigned)job.x >> 16);
     const int start = job.y;
-    const int len = job.z;
-    // Support segments of the job (points are sorted by support rank,
-    // meta.z >> 16; meta.z & 0xFFFF is the plane, whose footprint shape all
-    // planes of the segment share).
-    unsigned heads = 0;
-    int myplane[2] = {0, 0};
-    bool myhead[2] = {false, false};
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-    {
-        const int i = 2 * tid + q;
-        if (i < len)
-        {
-            const int z = meta[start + i].z;
-            myplane[q] = z & 0xFFFF;
-            myhead[q] = (i == 0) ||
-                    ((meta[start + i - 1].z >> 16) & kRankMask) !=
-                    ((z >> 16) & kRankMask);
-            heads += myhead[q] ? 1u : 0u;
-        }
-    }
-    unsigned nseg;
-    unsigned pos = block_exclusive_scan(heads, L.tmp, &nseg);
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-    {
-        if (myhead[q])
-        {
-            L.seg[pos] = (unsigned)(2 * tid + q);
-            L.segp[pos] = myplane[q];
-            ++pos;
-        }
-    }
-    if (tid == 0)
-    {
-        L.seg[nseg] = (unsigned)len;
-        L.nseg = nseg;
-    }
-    __syncthreads();
-    // Values read back from LDS are the same in every lane; make that
-    // explicit so that the point loop runs on scalar registers and scalar
-    // loads.
-    nseg = (unsigned)__builtin_amdgcn_readfirstlane((int)L.nseg);
+    // job.z: the item's points, and its support segments above bit 16
+    // (k_item_segments); segs[start + g] = segment g's first point in the
+    // item | its plane << 12 (planes of a segment share the footprint
+    // shape, hence the lane map).
+    const int len = job.z & 0xFFFF;
+    const unsigned nseg = (unsigned)job.z >> 16;
+    const ConstPtr<unsigned> sg = as_const<unsigned>(x.segs + start);
     for (unsigned g = 0; g < nseg; ++g)
     {
-        const int plane = __builtin_amdgcn_readfirstlane(L.segp[g]);
+        const unsigned sv = sg[g];
+        const int plane = (int)(sv >> 12);
         const PlaneInfo pi = ld_pinfo(pinfo, plane);
-        const int ra = __builtin_amdgcn_readfirstlane((int)L.seg[g]);
-        const int rb = __builtin_amdgcn_readfirstlane((int)L.seg[g + 1]);
+        const int ra = (int)(sv & 0xFFFu);
+        const int rb = g + 1 < nseg ? (int)(sg[g + 1] & 0xFFFu) : len;
         // streams = nwaves / wpp, a power of two (2^sshift).
         const int smask = (1 << pi.sshift) - 1;
         const int stream = wave / pi.wpp;
@@ -1555,11 +1560,11 @@ __device__ __forceinline__ void grid_work_item(GridLds& L, ScatterCtx<T>& x,
 // (workgroup b runs on XCD b mod xcds); XCD x takes the x-th of xcds
 // contiguous ranges from its own queue (s->item_next[x], zero at launch), so
 // its L2 holds the tap tables of that region only, then helps the other
-// queues in turn.  f calls pre() once, after its first barrier and before
-// its last one: lane 0 of the last wave then requests the next index of the
-// same queue into *s_next, and the index is read after f returns, so the
-// atomic's round trip overlaps the end of the item instead of preceding the
-// next one.  Every workgroup must call this (it synchronises).
+// queues in turn.  f calls pre() once, before its last barrier: lane 0 of
+// the last wave then requests the next index of the same queue into LDS,
+// and the index is read after f returns, so the atomic's round trip
+// overlaps the end of the item instead of preceding the next one.  f has at
+// least one barrier.  Every workgroup must call this (it synchronises).
 // f is called from one place only, so that it is inlined once: an
 // out-of-line f would receive the LDS region through a generic pointer and
 // its atomics would become flat atomics.
@@ -1567,6 +1572,9 @@ template <typename F>
 __device__ __forceinline__ void for_each_item(Summary* __restrict__ s,
         int xcds, unsigned* s_next, F&& f)
 {
+    // s_next[0..1]: the prefetched index by item parity (a slot is written
+    // again two items later, after a barrier every wave passes once it has
+    // read it); s_next[2]: take().
     const unsigned total = s->total_items;
     const bool queues = xcds > 1 && xcds <= kXcdGroups;
     const unsigned nq = queues ? (unsigned)xcds : 1u;
@@ -1582,10 +1590,10 @@ __device__ __forceinline__ void for_each_item(Summary* __restrict__ s,
             const unsigned lo = xq * per;
             const unsigned hi = min(total, lo + per);
             if (threadIdx.x == 0)
-                *s_next = lo + atomicAdd(&s->item_next[xq], 1u);
+                s_next[2] = lo + atomicAdd(&s->item_next[xq], 1u);
             __syncthreads();
             const unsigned got = (unsigned)__builtin_amdgcn_readfirstlane(
-                    (int)*s_next);
+                    (int)s_next[2]);
             __syncthreads();
             if (got < hi) return got;
         }
@@ -1598,18 +1606,21 @@ __device__ __forceinline__ void for_each_item(Summary* __restrict__ s,
         return;
     }
     unsigned it = take();
+    unsigned par = 0;
     while (it != ~0u)
     {
         const unsigned xq = (home + q) % nq;
         const unsigned lo = xq * per;
         const unsigned hi = min(total, lo + per);
         f(it, [&] {
-            if (leader) *s_next = lo + atomicAdd(&s->item_next[xq], 1u);
+            if (leader) s_next[par] = lo + atomicAdd(&s->item_next[xq], 1u);
         });
-        // f's last barrier orders the store above before this read, and
-        // f's first barrier orders this read before the next store.
+        // f's last barrier orders the store above before this read; the
+        // slot is stored again in the item after next, past the next
+        // item's last barrier, so after every wave's read.
         const unsigned got = (unsigned)__builtin_amdgcn_readfirstlane(
-                (int)*s_next);
+                (int)s_next[par]);
+        par ^= 1u;
         if (got < hi)
             it = got;
         else
@@ -1634,10 +1645,11 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
         const TapT* __restrict__ etab, int G, int row0,
         int R, T* __restrict__ grid, int force_direct, int min_share,
         unsigned* __restrict__ diag_count,
-        unsigned long long* __restrict__ diag_wsum, int xcds)
+        unsigned long long* __restrict__ diag_wsum, int xcds,
+        const unsigned* __restrict__ segs)
 {
     __shared__ GridLds L;
-    __shared__ unsigned s_next;
+    __shared__ unsigned s_next[3];
     const int tid = threadIdx.x;
     const int NT = blockDim.x;
     const bool lds = !force_direct && R > 0;
@@ -1645,6 +1657,7 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
     x.meta = meta;
     x.vq = vq;
     x.etab = etab;
+    x.segs = segs;
     x.lds = &L;
     x.grid = grid;
     x.diag_count = diag_count;
@@ -1658,7 +1671,7 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(
         L.re[i] = 0.0;
         L.im[i] = 0.0;
     }
-    for_each_item(summary, xcds, &s_next, [&](unsigned it, auto&& pre) {
+    for_each_item(summary, xcds, s_next, [&](unsigned it, auto&& pre) {
         grid_work_item<T, DIAG, TapT, COMP>(L, x, pinfo, items[it], lds,
                                             min_share, pre);
     });
@@ -2621,6 +2634,11 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
             (const unsigned long long*)mkeys, (const unsigned*)d_bstart_,
             (const unsigned*)d_bpos_, tile_, tile_shift_,
             (unsigned)kJobPoints, (unsigned)items_cap_, d_items_);
+    // Each item's support segments, into the run-head buffer (spent).
+    unsigned* const segs = mh;
+    hipLaunchKernelGGL(k_item_segments, dim3(blocks), dim3(256), 0, stream_,
+            (const Summary*)d_summary_, (unsigned)items_cap_, d_items_,
+            (const int4*)d_meta_, segs);
     check_hip(hipGetLastError(), kErrKernelLaunch, "job launch");
     check_hip(hipEventRecord(ev_[3], stream_), kErrKernelLaunch, "event");
     if (diag)
@@ -2629,7 +2647,8 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
                 d_summary_, (const int4*)d_items_,
                 (const PlaneInfo*)d_pinfo_, (const int4*)d_meta_,
                 (const T4*)d_vq_, (const T2*)d_etab_, G_, row0, 0, grid, 1,
-                min_share_, diag->per_cell, diag->wsum_cell, xcds_);
+                min_share_, diag->per_cell, diag->wsum_cell, xcds_,
+                (const unsigned*)segs);
     else
         hipLaunchKernelGGL((k_grid<T, false, TapT, COMP>), dim3(grid_blocks_),
                 dim3(kGridThreads), 0, stream_,
@@ -2637,7 +2656,7 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
                 (const PlaneInfo*)d_pinfo_, (const int4*)d_meta_,
                 (const T4*)d_vq_, (const TapT*)d_etab_, G_, row0, region_, grid,
                 0, min_share_, (unsigned*)nullptr,
-                (unsigned long long*)nullptr, xcds_);
+                (unsigned long long*)nullptr, xcds_, (const unsigned*)segs);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_grid launch");
     check_hip(hipEventRecord(ev_[4], stream_), kErrKernelLaunch, "event");
 }
@@ -3181,10 +3200,11 @@ __global__ void __launch_bounds__(256) k_gather_batch(
 __global__ void __launch_bounds__(kGridThreads) k_grid_batch(
         Summary* __restrict__ bsum, const int4* __restrict__ items,
         const BatchJob* __restrict__ jobs, const int4* __restrict__ meta,
-        const float4* __restrict__ vq, int xcds)
+        const float4* __restrict__ vq, int xcds,
+        const unsigned* __restrict__ segs)
 {
     __shared__ GridLds L;
-    __shared__ unsigned s_next;
+    __shared__ unsigned s_next[3];
     for (int i = threadIdx.x; i < kRegionCells; i += blockDim.x)
     {
         L.re[i] = 0.0;
@@ -3194,11 +3214,12 @@ __global__ void __launch_bounds__(kGridThreads) k_grid_batch(
     ScatterCtx<float> x;
     x.meta = meta;
     x.vq = vq;
+    x.segs = segs;
     x.lds = &L;
     x.diag_count = nullptr;
     x.diag_wsum = nullptr;
     x.row0 = 0;
-    for_each_item(bsum, xcds, &s_next, [&](unsigned it, auto&& pre) {
+    for_each_item(bsum, xcds, s_next, [&](unsigned it, auto&& pre) {
         const int4 item = items[it];
         // BatchJob fields by word: pinfo, etab, grid (2 words each), G, R,
         // tile_bits, tile_shift, key_bits, min_share, ...
@@ -3562,10 +3583,14 @@ void Batch::enqueue()
             (unsigned)tiles_total_, (const unsigned*)d_tstart_,
             (const unsigned*)d_tend_, (const unsigned*)d_ioff_,
             (const BatchJob*)d_desc_, (int)J, (unsigned)items_cap_, d_items_);
+    // Each item's support segments, into the run-head buffer (spent).
+    hipLaunchKernelGGL(k_item_segments, dim3(pblocks), dim3(256), 0, s,
+            (const Summary*)d_bsum_, (unsigned)items_cap_, d_items_,
+            (const int4*)d_meta_, mh);
     hipLaunchKernelGGL(k_grid_batch, dim3(grid_blocks_), dim3(kGridThreads),
             0, s, d_bsum_, (const int4*)d_items_,
             (const BatchJob*)d_desc_, (const int4*)d_meta_,
-            (const float4*)d_vq_, kXcdGroups);
+            (const float4*)d_vq_, kXcdGroups, (const unsigned*)mh);
     check_hip(hipGetLastError(), kErrKernelLaunch, "batch launch");
 }
 
