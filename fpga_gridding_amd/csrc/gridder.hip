@@ -823,22 +823,31 @@ __global__ void __launch_bounds__(256) k_item_segments(
         const int4 job = items[it];
         const int f = job.y, len = job.z & 0xFFFF;
         unsigned cnt = 0;
-        for (int b = 0; b < len; b += 64)
+        unsigned carry = 0;         // rank field of the previous chunk's end
+        for (int b = 0; b < len; b += 4 * 64)
         {
-            const int i = b + lane;
-            bool head = false;
-            unsigned z = 0;
-            if (i < len)
+            unsigned z[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
             {
-                z = (unsigned)meta[f + i].z;
-                head = i == 0 || (((unsigned)meta[f + i - 1].z >> 16) &
-                        kRankMask) != ((z >> 16) & kRankMask);
+                const int i = b + u * 64 + lane;
+                z[u] = i < len ? (unsigned)meta[f + i].z : 0u;
             }
-            const unsigned long long m = __ballot(head);
-            if (head)
-                segs[f + cnt + (unsigned)__popcll(m & below)] =
-                        (unsigned)i | ((z & 0xFFFFu) << 12);
-            cnt += (unsigned)__popcll(m);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                const int i = b + u * 64 + lane;
+                const unsigned r = (z[u] >> 16) & kRankMask;
+                unsigned prev = (unsigned)__shfl_up((int)r, 1);
+                if (lane == 0) prev = carry;
+                const bool head = i < len && (i == 0 || prev != r);
+                const unsigned long long m = __ballot(head);
+                if (head)
+                    segs[f + cnt + (unsigned)__popcll(m & below)] =
+                            (unsigned)i | ((z[u] & 0xFFFFu) << 12);
+                cnt += (unsigned)__popcll(m);
+                carry = (unsigned)__shfl((int)r, 63);
+            }
         }
         if (lane == 0) items[it].z = len | (int)(cnt << 16);
     }
