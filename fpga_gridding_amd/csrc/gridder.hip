@@ -7,9 +7,10 @@
 //   k_locate   per point: the reference's index math, bit-exact
 //              (oskar_grid_wproj.c:169-212), skip count, per-(plane,
 //              sub-cell) histogram, footprint bounding box, a sort key
-//              (tile of the footprint origin, plane, origin inside the
-//              tile, sub-cell table) and a 16/32-byte point record;
-//   sort       radix sort of the keys (rocPRIM);
+//              (tile of the footprint origin, support rank, origin inside
+//              the tile, plane, sub-cell table) and a 16/32-byte point
+//              record;
+//   sort       the library's stable LSD radix sort of the keys;
 //   k_plan     points gridded, update count and norm increment (histogram x
 //              per-(plane, sub-cell) tap sums, fixed summation order);
 //   k_gather   sorted point records (origin, sub-cell table, plane; the
@@ -17,13 +18,13 @@
 //              run of equal keys (same origin, plane and sub-cell table:
 //              the run's visibility terms summed), and the heads of the
 //              tile buckets;
-//   scans      bucket starts and jobs (<= kJobPoints points of one tile);
+//   scans      bucket starts and jobs (<= kJobPoints points of one tile),
+//              then each job's support segments (k_item_segments);
 //   k_grid     the scatter, one job per workgroup, into an FP64 LDS copy of
 //              the tile's grid region, added to the grid in HBM once per job.
 
 #include "gridder_internal.h"
 #include "host_semantics.h"
-
 
 #include <algorithm>
 #include <cstdlib>
