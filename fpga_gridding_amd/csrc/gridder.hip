@@ -650,32 +650,55 @@ __device__ __forceinline__ int region_offset(int pos, int R, int tile)
             (int)sizeof(double);
 }
 
-// Runs of equal sort keys: mh[s] = 1 where sorted point s starts one (its
-// key differs from the previous point's), or everywhere when merge == 0.
-// Points of one run share the tile, footprint origin, plane and sub-cell
-// table, so they use the same taps at the same cells: the scatter takes
-// one record per run, its visibility terms summed (k_gather).
+// Runs of equal sort keys, as bits: bit (s & 31) of hb[s >> 5] is set where
+// sorted point s starts a run (merge: its key differs from the previous
+// one; otherwise every point), hc[w] = the set bits of word w, and
+// summary->nw32 = the words of the summary's nv32 points.  Each wave takes
+// 64 consecutive points (two words) with one ballot.  Points of one run
+// share the tile, footprint origin, plane and sub-cell table, so they use
+// the same taps at the same cells: the scatter takes one record per run,
+// its visibility terms summed (k_gather).
 __global__ void __launch_bounds__(256) k_merge_heads(
-        const Summary* __restrict__ summary,
+        Summary* __restrict__ summary,
         const unsigned long long* __restrict__ keys,
-        unsigned* __restrict__ mh, int merge)
+        unsigned* __restrict__ hb, unsigned* __restrict__ hc, int merge)
 {
     const unsigned nv = summary->nv32;
-    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
+    const unsigned nw = (nv + 31) / 32;
+    if (blockIdx.x == 0 && threadIdx.x == 0) summary->nw32 = nw;
+    const int lane = threadIdx.x & 63;
+    const unsigned end = (nv + 63) & ~63u;     // whole waves
+    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < end;
          s += gridDim.x * blockDim.x)
-        mh[s] = (!merge || s == 0 || keys[s - 1] != keys[s]) ? 1u : 0u;
+    {
+        const bool h = s < nv &&
+                (!merge || s == 0 || keys[s - 1] != keys[s]);
+        const unsigned long long m = __ballot(h);
+        const unsigned w = s >> 5;
+        if ((lane & 31) == 0 && w < nw)
+        {
+            const unsigned bits = (unsigned)(lane == 0 ? m : m >> 32);
+            hb[w] = bits;
+            hc[w] = (unsigned)__popc(bits);
+        }
+    }
 }
 
-// rstart[m] = first sorted point of run m (mh, run starts; mpos, their
-// exclusive scan); rstart[runs] = the point count.
+// rstart[m] = first sorted point of run m (hb, the run-start bits; hp, the
+// exclusive scan of their per-word counts); rstart[runs] = the point count.
 __global__ void __launch_bounds__(256) k_run_starts(
-        const Summary* __restrict__ summary, const unsigned* __restrict__ mh,
-        const unsigned* __restrict__ mpos, unsigned* __restrict__ rstart)
+        const Summary* __restrict__ summary, const unsigned* __restrict__ hb,
+        const unsigned* __restrict__ hp, unsigned* __restrict__ rstart)
 {
     const unsigned nv = summary->nv32;
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
          s += gridDim.x * blockDim.x)
-        if (mh[s]) rstart[mpos[s]] = s;
+    {
+        const unsigned bits = hb[s >> 5];
+        const unsigned bit = 1u << (s & 31);
+        if (bits & bit)
+            rstart[hp[s >> 5] + (unsigned)__popc(bits & (bit - 1u))] = s;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         rstart[summary->nm32] = nv;
 }
@@ -2615,13 +2638,17 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     // into the sort's spent key buffer.
     unsigned* const mh = vv[cur ^ 1];
     unsigned long long* const mkeys = kk[cur ^ 1];
+    const size_t nwmax = n / 32 + 1;
+    unsigned* const hb = mh;                 // run-start bits
+    unsigned* const hc = hb + nwmax;         // their counts per word
+    unsigned* const hp = hc + nwmax;         // the counts' exclusive scan
     hipLaunchKernelGGL(k_merge_heads, dim3(blocks), dim3(256), 0, stream_,
-            (const Summary*)d_summary_, skeys, mh, diag ? 0 : 1);
-    scan_u32(mh, d_bpos_, &d_summary_->nv32, n, &d_summary_->nm32);
+            d_summary_, skeys, hb, hc, diag ? 0 : 1);
+    scan_u32(hc, hp, &d_summary_->nw32, nwmax, &d_summary_->nm32);
     // Run starts into the bucket table (free until the tile scan below).
     hipLaunchKernelGGL(k_run_starts, dim3(blocks), dim3(256), 0, stream_,
-            (const Summary*)d_summary_, (const unsigned*)mh,
-            (const unsigned*)d_bpos_, d_bstart_);
+            (const Summary*)d_summary_, (const unsigned*)hb,
+            (const unsigned*)hp, d_bstart_);
     hipLaunchKernelGGL((k_gather<T>), dim3(blocks), dim3(256), 0, stream_,
             d_summary_, skeys, sidx, (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
             (const PlaneInfo*)d_pinfo_, h_, K_, tile_shift_, d_meta_,
@@ -3305,7 +3332,7 @@ void Batch::free_buffers()
                     (void*)d_idx_[1], d_rec_, (void*)d_meta_, d_vq_,
                     (void*)d_tstart_, (void*)d_tend_, (void*)d_icnt_,
                     (void*)d_ioff_, (void*)d_part_, (void*)d_rhist_,
-                    (void*)d_roff_, (void*)d_items_, (void*)d_mpos_,
+                    (void*)d_roff_, (void*)d_items_,
                     (void*)d_rstart_})
         dfree(p);
     if (h_sum_) (void)hipHostFree(h_sum_);
@@ -3319,7 +3346,7 @@ void Batch::free_buffers()
     d_vq_ = nullptr;
     d_tstart_ = d_tend_ = d_icnt_ = d_ioff_ = d_part_ = nullptr;
     d_rhist_ = d_roff_ = nullptr;
-    d_mpos_ = d_rstart_ = nullptr;
+    d_rstart_ = nullptr;
     d_items_ = nullptr;
     h_sum_ = nullptr;
     cap_points_ = cap_tiles_ = cap_combo_ = cap_jobs_ = 0;
@@ -3437,7 +3464,6 @@ void Batch::prepare()
         d_vq_ = dmalloc<float4>(cap_points_);
         d_tstart_ = dmalloc<unsigned>(cap_tiles_);
         d_tend_ = dmalloc<unsigned>(cap_tiles_);
-        d_mpos_ = dmalloc<unsigned>(cap_points_);
         d_rstart_ = dmalloc<unsigned>(cap_points_ + 1);
         const size_t rh = radix_hist_words(cap_points_);
         d_rhist_ = dmalloc<unsigned>(rh);
@@ -3570,13 +3596,17 @@ void Batch::enqueue()
     unsigned long long* const mkeys = kk[1 - cur];
     hipLaunchKernelGGL(k_batch_count, dim3(pblocks), dim3(256), 0, s,
             n_total_, kin, lmax_, (unsigned)tiles_total_, d_bsum_);
+    const size_t nwmax = n_total_ / 32 + 1;
+    unsigned* const hb = mh;                 // run-start bits
+    unsigned* const hc = hb + nwmax;         // their counts per word
+    unsigned* const hp = hc + nwmax;         // the counts' exclusive scan
     hipLaunchKernelGGL(k_merge_heads, dim3(pblocks), dim3(256), 0, s,
-            (const Summary*)d_bsum_, kin, mh, 1);
-    scan_u32_enqueue(mh, d_mpos_, &d_bsum_->nv32, n_total_, &d_bsum_->nm32,
+            d_bsum_, kin, hb, hc, 1);
+    scan_u32_enqueue(hc, hp, &d_bsum_->nw32, nwmax, &d_bsum_->nm32,
             d_part_, s);
     hipLaunchKernelGGL(k_run_starts, dim3(pblocks), dim3(256), 0, s,
-            (const Summary*)d_bsum_, (const unsigned*)mh,
-            (const unsigned*)d_mpos_, d_rstart_);
+            (const Summary*)d_bsum_, (const unsigned*)hb,
+            (const unsigned*)hp, d_rstart_);
     hipLaunchKernelGGL(k_gather_batch, dim3(pblocks), dim3(256), 0, s,
             (const Summary*)d_bsum_, kin, vin, (const RecF*)d_rec_,
             (const BatchJob*)d_desc_, (int)J, lmax_, d_meta_,

@@ -80,6 +80,7 @@ struct Summary
                                     // keys (nv32 without merging)
     unsigned long long run_updates; // sum over records of W^2: the grid
                                     // updates the scatter executes
+    unsigned nw32;                  // 32-point words of run-start bits
 };
 
 // Per-point record written by the locate kernel in input order.
@@ -489,7 +490,6 @@ private:
     void* d_vq_ = nullptr;
     unsigned* d_tstart_ = nullptr;    // per global tile: first record
     unsigned* d_tend_ = nullptr;      // one past the last
-    unsigned* d_mpos_ = nullptr;      // per sorted point: index of its run
                                       // (runs of equal keys: one record)
     unsigned* d_rstart_ = nullptr;    // per run: first sorted point
     unsigned* d_icnt_ = nullptr;
