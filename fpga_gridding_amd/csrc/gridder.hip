@@ -612,8 +612,9 @@ __global__ void __launch_bounds__(1024) k_plan(
 // (off_v + h) * K + (off_u + h), plane, grid_w); vq = (re, -s*im, im, s*re)
 // with s = -1 when ww > 0 (oskar_grid_wproj.c:172, :233), so that
 //   grid_re += vq.x * c_re + vq.y * c_im,  grid_im += vq.z * c_re + vq.w * c_im
-// with (c_re, c_im) the tap as stored in the kernel cube.  head[m] = 1 where
-// record m starts a new tile (key bits above tile_shift).
+// with (c_re, c_im) the tap as stored in the kernel cube.  Bit (m & 31) of
+// tb[m >> 5] is set where record m starts a new tile (key bits above
+// tile_shift), tc[w] = the set bits of word w, summary->nwt32 = the words.
 // meta.y of a point: bits 0-29 the element offset of its (plane, |off_v|,
 // |off_u|) tap table (tables are below 4 GiB and taps at least 4 bytes),
 // bit 30 set for off_u < 0, bit 31 for off_v < 0 (the scatter then reads
@@ -718,12 +719,16 @@ __global__ void __launch_bounds__(256) k_gather(
         const typename Types<T>::Rec* __restrict__ rec,
         const unsigned* __restrict__ gw, const PlaneInfo* __restrict__ pinfo,
         int h, int K, int tile_shift, int4* __restrict__ meta,
-        typename Types<T>::T4* __restrict__ vq, unsigned* __restrict__ head,
-        int R, int tile, int pos_shift, const unsigned* __restrict__ rstart,
+        typename Types<T>::T4* __restrict__ vq, unsigned* __restrict__ tb,
+        unsigned* __restrict__ tc, int R, int tile, int pos_shift,
+        const unsigned* __restrict__ rstart,
         unsigned long long* __restrict__ mkeys)
 {
     typedef typename Types<T>::Rec Rec;
     const unsigned nm = summary->nm32;
+    const unsigned nwt = (nm + 31) / 32;
+    if (blockIdx.x == 0 && threadIdx.x == 0) summary->nwt32 = nwt;
+    const int lane = threadIdx.x & 63;
     unsigned long long upd = 0;     // footprint cells of this thread's runs
     for (unsigned m = blockIdx.x * blockDim.x + threadIdx.x; m < nm;
          m += gridDim.x * blockDim.x)
@@ -731,8 +736,16 @@ __global__ void __launch_bounds__(256) k_gather(
         const unsigned s = rstart[m], e = rstart[m + 1];
         const unsigned long long key = keys[s];
         mkeys[m] = key;
-        head[m] = (s == 0 || (keys[s - 1] >> tile_shift) !=
-                (key >> tile_shift)) ? 1u : 0u;
+        // 64 consecutive records per wave (lanes past nm have left the
+        // loop and count as zero bits).
+        const unsigned long long tbits = __ballot(s == 0 ||
+                (keys[s - 1] >> tile_shift) != (key >> tile_shift));
+        if ((lane & 31) == 0 && (m >> 5) < nwt)
+        {
+            const unsigned bits = (unsigned)(lane == 0 ? tbits : tbits >> 32);
+            tb[m >> 5] = bits;
+            tc[m >> 5] = (unsigned)__popc(bits);
+        }
         const unsigned i = idx[s];
         const Rec r = rec[i];
         const int plane = (unsigned)r.sub >> 17;
@@ -792,13 +805,18 @@ __global__ void __launch_bounds__(256) k_gather(
 
 // bstart[b] = first record of tile bucket b; bstart[n_buckets] = records.
 __global__ void __launch_bounds__(256) k_buckets(
-        const Summary* __restrict__ summary, const unsigned* __restrict__ head,
-        const unsigned* __restrict__ bpos, unsigned* __restrict__ bstart)
+        const Summary* __restrict__ summary, const unsigned* __restrict__ tb,
+        const unsigned* __restrict__ tp, unsigned* __restrict__ bstart)
 {
     const unsigned nv = summary->nm32;
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nv;
          s += gridDim.x * blockDim.x)
-        if (head[s]) bstart[bpos[s]] = s;
+    {
+        const unsigned bits = tb[s >> 5];
+        const unsigned bit = 1u << (s & 31);
+        if (bits & bit)
+            bstart[tp[s >> 5] + (unsigned)__popc(bits & (bit - 1u))] = s;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         bstart[summary->n_buckets] = nv;
 }
@@ -2649,22 +2667,26 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     hipLaunchKernelGGL(k_run_starts, dim3(blocks), dim3(256), 0, stream_,
             (const Summary*)d_summary_, (const unsigned*)hb,
             (const unsigned*)hp, d_bstart_);
+    unsigned* const tb = d_head_;            // tile-start bits
+    unsigned* const tc = tb + nwmax;         // their counts per word
     hipLaunchKernelGGL((k_gather<T>), dim3(blocks), dim3(256), 0, stream_,
             d_summary_, skeys, sidx, (const Rec*)d_rec_, (const unsigned*)(diag ? d_gw_ : nullptr),
             (const PlaneInfo*)d_pinfo_, h_, K_, tile_shift_, d_meta_,
-            (T4*)d_vq_, d_head_, diag ? 0 : region_, tile_,
+            (T4*)d_vq_, tb, tc, diag ? 0 : region_, tile_,
             combo_bits_ + plane_bits_, (const unsigned*)d_bstart_, mkeys);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_gather launch");
-    // Tiles: exclusive scan of the heads, then their starts.
-    scan_u32(d_head_, d_bpos_, &d_summary_->nm32, n, &d_summary_->n_buckets);
+    // Tiles: exclusive scan of the tile-start bits' counts, then the
+    // tiles' first records.
+    scan_u32(tc, d_bpos_, &d_summary_->nwt32, nwmax, &d_summary_->n_buckets);
     hipLaunchKernelGGL(k_buckets, dim3(blocks), dim3(256), 0, stream_,
-            (const Summary*)d_summary_, (const unsigned*)d_head_,
+            (const Summary*)d_summary_, (const unsigned*)tb,
             (const unsigned*)d_bpos_, d_bstart_);
     // Jobs per tile (reusing head / bpos), scanned to offsets.
     hipLaunchKernelGGL(k_bucket_items, dim3(blocks), dim3(256), 0, stream_,
             (const Summary*)d_summary_, (const unsigned*)d_bstart_,
             (unsigned)kJobPoints, d_head_);
-    scan_u32(d_head_, d_bpos_, &d_summary_->n_buckets, n,
+    scan_u32(d_head_, d_bpos_, &d_summary_->n_buckets,
+            std::min<size_t>(n, (size_t)1 << (2 * tile_coord_bits_)),
             &d_summary_->total_items);
     hipLaunchKernelGGL(k_write_items, dim3(blocks), dim3(256), 0, stream_,
             d_summary_,

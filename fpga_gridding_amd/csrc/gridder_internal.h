@@ -81,6 +81,7 @@ struct Summary
     unsigned long long run_updates; // sum over records of W^2: the grid
                                     // updates the scatter executes
     unsigned nw32;                  // 32-point words of run-start bits
+    unsigned nwt32;                 // 32-record words of tile-start bits
 };
 
 // Per-point record written by the locate kernel in input order.
