@@ -47,14 +47,6 @@ void check_hip(hipError_t e, int code, const char* what)
                 hipGetErrorString(e) + ")");
 }
 
-// Bits needed to hold values 0..n-1 (at least 1).
-__host__ __device__ __forceinline__ int bits_plane_(int n)
-{
-    int b = 1;
-    while ((1 << b) < n) ++b;
-    return b;
-}
-
 __device__ __forceinline__ unsigned long long spread_bits16(unsigned x)
 {
     unsigned long long v = x & 0xFFFFull;
@@ -260,7 +252,7 @@ __global__ void __launch_bounds__(256) k_locate(
         size_t n, const TU* __restrict__ uu, const TU* __restrict__ vv,
         const TU* __restrict__ ww, const TV* __restrict__ vis, T grid_scale,
         T w_scale, int ov, int G, int P, const int* __restrict__ support,
-        const int* __restrict__ srank, int srank_bits,
+        const int* __restrict__ srank, int srank_bits, int plane_bits,
         int h, int K, int tile, int tile_bits, int combo_bits, int key_bits,
         unsigned long long* __restrict__ keys, unsigned* __restrict__ idx,
         Rec* __restrict__ rec,
@@ -287,8 +279,12 @@ __global__ void __launch_bounds__(256) k_locate(
     // so the scatter takes them as one segment, and points at one position
     // share its running sums across planes and sub-cell tables.
     const unsigned long long skip_key = 1ull << key_bits;
+    // srank[p] = the plane's support rank | its index among the planes of
+    // that rank << 16: the key's plane field below the position need only
+    // tell apart planes of one support, since the rank above it does the
+    // rest (plane_bits wide).
     const int sh_plane = combo_bits;
-    const int sh_pos = sh_plane + bits_plane_(P);
+    const int sh_pos = sh_plane + plane_bits;
     const int sh_srank = sh_pos + 2 * tile_bits;
     const int sh_tile = sh_srank + srank_bits;
     int nskip = 0;
@@ -346,10 +342,11 @@ __global__ void __launch_bounds__(256) k_locate(
                     (spread_bits16(tv) << 1);
             const unsigned long long pz = spread_bits16(ou) |
                     (spread_bits16(ov) << 1);
+            const unsigned sr = (unsigned)srank[plane];
             key = (tz << sh_tile) |
-                    ((unsigned long long)srank[plane] << sh_srank) |
+                    ((unsigned long long)(sr & 0xFFFFu) << sh_srank) |
                     (pz << sh_pos) |
-                    ((unsigned long long)plane << sh_plane) |
+                    ((unsigned long long)(sr >> 16) << sh_plane) |
                     (unsigned long long)table;
             if (lds_hist) atomicAdd(&s_combo[c], 1u);
             else atomicAdd(&combo[c], 1u);
@@ -1989,7 +1986,6 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
     gscale_d_ = (double)grid_size * cell_d_;
     coord_bits_ = std::max(1, bits_for((unsigned long long)(G_ - 1)));
     combo_bits_ = std::max(1, bits_for((unsigned long long)(K_ * K_ - 1)));
-    plane_bits_ = bits_plane_((int)P_);
 
     // Support ranks (distinct S values in increasing order).
     std::vector<int> svals(support_);
@@ -2001,6 +1997,20 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
                 support_[p]) - svals.begin());
     srank_bits_ = std::max(1, bits_for((unsigned long long)(svals.size() -
             1)));
+    // Each plane's index among the planes of its support rank (in plane
+    // order), packed with the rank for the index stage's keys.
+    srank_key_.resize(P_);
+    {
+        std::vector<int> seen(svals.size(), 0);
+        int most = 1;
+        for (size_t p = 0; p < P_; ++p)
+        {
+            const int local = seen[srank_[p]]++;
+            most = std::max(most, local + 1);
+            srank_key_[p] = srank_[p] | (local << 16);
+        }
+        plane_bits_ = bits_for((unsigned long long)(most - 1));
+    }
     // Tile and in-tile coordinates take at most one bit more per axis
     // than a grid coordinate (one each for a side that is not a power of
     // two); configure() sets the exact width for the chosen side.
@@ -2114,7 +2124,7 @@ Plan::Plan(int precision, size_t num_w_planes, const int* support,
             hipMemcpyHostToDevice), kErrMemCopy, "upload tap tables");
     check_hip(hipMemcpy(d_support_, support_.data(), P_ * sizeof(int),
             hipMemcpyHostToDevice), kErrMemCopy, "upload support");
-    check_hip(hipMemcpy(d_srank_, srank_.data(), P_ * sizeof(int),
+    check_hip(hipMemcpy(d_srank_, srank_key_.data(), P_ * sizeof(int),
             hipMemcpyHostToDevice), kErrMemCopy, "upload support ranks");
     check_hip(hipMemcpy(d_pinfo_, pinfo_.data(), P_ * sizeof(PlaneInfo),
             hipMemcpyHostToDevice), kErrMemCopy, "upload plane table");
@@ -2157,7 +2167,7 @@ void Plan::build_tables_device(const float* d_conv)
     std::memset(h_summary_, 0, sizeof(Summary));
     check_hip(hipMemcpy(d_support_, support_.data(), P_ * sizeof(int),
             hipMemcpyHostToDevice), kErrMemCopy, "upload support");
-    check_hip(hipMemcpy(d_srank_, srank_.data(), P_ * sizeof(int),
+    check_hip(hipMemcpy(d_srank_, srank_key_.data(), P_ * sizeof(int),
             hipMemcpyHostToDevice), kErrMemCopy, "upload support ranks");
     check_hip(hipMemcpy(d_pinfo_, pinfo_.data(), P_ * sizeof(PlaneInfo),
             hipMemcpyHostToDevice), kErrMemCopy, "upload plane table");
@@ -2608,7 +2618,8 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
     if (diag)
         hipLaunchKernelGGL((k_locate<T, true, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
-                d_support_, (const int*)d_srank_, srank_bits_, h_, K_,
+                d_support_, (const int*)d_srank_, srank_bits_, plane_bits_,
+                h_, K_,
                 tile_, tile_bits_, combo_bits_, key_bits_,
                 d_keys_[0], d_idx_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, d_gw_, diag->per_support,
@@ -2616,7 +2627,8 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
     else
         hipLaunchKernelGGL((k_locate<T, false, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
-                d_support_, (const int*)d_srank_, srank_bits_, h_, K_,
+                d_support_, (const int*)d_srank_, srank_bits_, plane_bits_,
+                h_, K_,
                 tile_, tile_bits_, combo_bits_, key_bits_,
                 d_keys_[0], d_idx_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, (unsigned*)nullptr,
@@ -3564,7 +3576,8 @@ void Batch::enqueue()
                         dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
                         e.vis, pl.grid_scale_d(), pl.w_scale_d(),
                         pl.oversample(), d.G, P, pl.device_support(),
-                        pl.device_srank(), pl.srank_bits(), d.sub_h,
+                        pl.device_srank(), pl.srank_bits(), pl.plane_bits(),
+                        d.sub_h,
                         K, d.tile, d.tile_bits, pl.combo_bits(), d.key_bits,
                         d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
                         static_cast<RecF*>(d_rec_) + d.point_base, combo,
@@ -3578,7 +3591,8 @@ void Batch::enqueue()
                         dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
                         e.vis, pl.grid_scale_f(), pl.w_scale_f(),
                         pl.oversample(), d.G, P, pl.device_support(),
-                        pl.device_srank(), pl.srank_bits(), d.sub_h,
+                        pl.device_srank(), pl.srank_bits(), pl.plane_bits(),
+                        d.sub_h,
                         K, d.tile, d.tile_bits, pl.combo_bits(), d.key_bits,
                         d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
                         static_cast<RecF*>(d_rec_) + d.point_base, combo,

@@ -279,6 +279,7 @@ public:
     const int* device_support() const { return d_support_; }
     const int* device_srank() const { return d_srank_; }
     int srank_bits() const { return srank_bits_; }
+    int plane_bits() const { return plane_bits_; }
     int coord_bits() const { return coord_bits_; }
     int combo_bits() const { return combo_bits_; }
     int tile_shift() const { return tile_shift_; }
@@ -329,7 +330,7 @@ private:
     double cell_d_, ws_d_, gscale_d_;
     int coord_bits_;     // bits of a grid coordinate
     int combo_bits_;     // bits of a sub-cell table index
-    int plane_bits_;     // bits of a plane index
+    int plane_bits_;     // bits of a plane's index among its rank's planes
     int srank_bits_;     // bits of a support rank
     int tile_;           // tile side T (1..64)
     int tile_bits_;      // bits of a coordinate inside a tile (T <= 2^bits)
@@ -340,6 +341,7 @@ private:
     int region_;         // LDS region side T + Wmax - 1 (0: none)
     std::vector<int> support_;
     std::vector<int> srank_;
+    std::vector<int> srank_key_;  // srank_ | index among the rank's planes << 16
     std::vector<PlaneInfo> pinfo_;
     int min_share_ = kMinShare; // least points per scatter stream
     int xcds_ = kXcdGroups;     // scatter item queues (1: grid-stride order)
