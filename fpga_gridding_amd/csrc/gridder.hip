@@ -2663,9 +2663,9 @@ void Plan::scatter_t(size_t n, T* grid, int row0, const DiagBuffers* diag)
     const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
             (n + 255) / 256, (size_t)n_cu_ * 4));
     // Runs of equal keys -> one record each (not for the diagnostics, which
-    // count points per cell): run heads into the sort's spent value buffer,
-    // their scan into bpos (free until the tile scan below), the runs' keys
-    // into the sort's spent key buffer.
+    // count points per cell): the run-start bits, their counts and the
+    // counts' scan into the sort's spent value buffer, the runs' keys into
+    // its spent key buffer.
     unsigned* const mh = vv[cur ^ 1];
     unsigned long long* const mkeys = kk[cur ^ 1];
     const size_t nwmax = n / 32 + 1;
@@ -3626,8 +3626,9 @@ void Batch::enqueue()
     const unsigned long long* kin = kk[cur];
     const unsigned* vin = vv[cur];
     // Runs of equal keys (same job, tile, origin, plane and sub-cell table)
-    // -> one record each: run heads into the sort's spent value buffer, the
-    // runs' keys into its spent key buffer; tiles over the run records.
+    // -> one record each: the run-start bits, their counts and the counts'
+    // scan into the sort's spent value buffer, the runs' keys into its spent
+    // key buffer; tiles over the run records.
     unsigned* const mh = vv[1 - cur];
     unsigned long long* const mkeys = kk[1 - cur];
     hipLaunchKernelGGL(k_batch_count, dim3(pblocks), dim3(256), 0, s,
