@@ -240,6 +240,12 @@ __global__ void k_fill_u32(unsigned* __restrict__ p, size_t n)
 }
 
 constexpr int kComboLds = 8192;   // per-block LDS histogram capacity
+// k_locate: one 1024-thread workgroup per CU (grid-stride).  Every
+// workgroup adds its (plane, sub-cell) histogram into the same global bins
+// at the end, so fewer, larger workgroups mean fewer atomics on each bin
+// (measured: the 64-job mix's graph 32.7 -> 31.0 ms against 256-thread
+// workgroups, four per CU; the 512-station index stage unchanged).
+constexpr int kLocateThreads = 1024;
 
 // Index stage.  Follows oskar_grid_wproj.c:169-212 (FP32) and :71-101
 // (FP64) operation by operation with FMA contraction disabled.  T is the
@@ -248,7 +254,7 @@ constexpr int kComboLds = 8192;   // per-block LDS histogram capacity
 // coordinates, FP16/BF16 visibilities, FP64 index math, float records).
 template <typename T, bool DIAG, typename TU = T, typename TV = T,
           typename Rec = typename Types<T>::Rec>
-__global__ void __launch_bounds__(256) k_locate(
+__global__ void __launch_bounds__(kLocateThreads) k_locate(
         size_t n, const TU* __restrict__ uu, const TU* __restrict__ vv,
         const TU* __restrict__ ww, const TV* __restrict__ vis, T grid_scale,
         T w_scale, int ov, int G, int P, const int* __restrict__ support,
@@ -264,7 +270,7 @@ __global__ void __launch_bounds__(256) k_locate(
 {
 #pragma clang fp contract(off)
     __shared__ unsigned s_combo[kComboLds];
-    __shared__ int s_red[4][5];
+    __shared__ int s_red[kLocateThreads / 64][5];
     const int n_combo = P * K * K;
     const bool lds_hist = n_combo <= kComboLds;
     if (lds_hist)
@@ -2612,11 +2618,12 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
             (n_combo + 255) / 256))), dim3(256), 0, stream_, d_summary_,
             d_combo_, n_combo);
     const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
-            (n + 255) / 256, (size_t)n_cu_ * 4));
+            (n + kLocateThreads - 1) / kLocateThreads, (size_t)n_cu_));
     const T gscale = sizeof(T) == 4 ? (T)gscale_f_ : (T)gscale_d_;
     const T wsc = sizeof(T) == 4 ? (T)ws_f_ : (T)ws_d_;
     if (diag)
-        hipLaunchKernelGGL((k_locate<T, true, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
+        hipLaunchKernelGGL((k_locate<T, true, TU, TV, Rec>), dim3(blocks),
+                dim3(kLocateThreads), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
                 d_support_, (const int*)d_srank_, srank_bits_, plane_bits_,
                 h_, K_,
@@ -2625,7 +2632,8 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
                 (Rec*)d_rec_, d_combo_, d_summary_, d_gw_, diag->per_support,
                 diag->per_plane, diag->plane_cap, 0ull, 0u);
     else
-        hipLaunchKernelGGL((k_locate<T, false, TU, TV, Rec>), dim3(blocks), dim3(256), 0,
+        hipLaunchKernelGGL((k_locate<T, false, TU, TV, Rec>), dim3(blocks),
+                dim3(kLocateThreads), 0,
                 stream_, n, uu, vv, ww, vis, gscale, wsc, ov_, G_, (int)P_,
                 d_support_, (const int*)d_srank_, srank_bits_, plane_bits_,
                 h_, K_,
@@ -3569,11 +3577,12 @@ void Batch::enqueue()
         if (e.n)
         {
             const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
-                    (e.n + 255) / 256, (size_t)n_cu_ * 4));
+                    (e.n + kLocateThreads - 1) / kLocateThreads,
+                    (size_t)n_cu_));
             if (pl.index64())
                 hipLaunchKernelGGL(
                         (k_locate<double, false, float, float, RecF>),
-                        dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
+                        dim3(blocks), dim3(kLocateThreads), 0, s, e.n, e.uu, e.vv, e.ww,
                         e.vis, pl.grid_scale_d(), pl.w_scale_d(),
                         pl.oversample(), d.G, P, pl.device_support(),
                         pl.device_srank(), pl.srank_bits(), pl.plane_bits(),
@@ -3588,7 +3597,7 @@ void Batch::enqueue()
                         d.point_base);
             else
                 hipLaunchKernelGGL((k_locate<float, false, float, float, RecF>),
-                        dim3(blocks), dim3(256), 0, s, e.n, e.uu, e.vv, e.ww,
+                        dim3(blocks), dim3(kLocateThreads), 0, s, e.n, e.uu, e.vv, e.ww,
                         e.vis, pl.grid_scale_f(), pl.w_scale_f(),
                         pl.oversample(), d.G, P, pl.device_support(),
                         pl.device_srank(), pl.srank_bits(), pl.plane_bits(),
