@@ -200,29 +200,34 @@ __device__ __forceinline__ int wave_max(int v)
 // Kernels
 // -------------------------------------------------------------------------
 
-__global__ void k_init(Summary* __restrict__ s, unsigned* __restrict__ combo,
-                       int n_combo)
+__device__ __forceinline__ void init_summary(Summary* __restrict__ s)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0)
-    {
-        s->skipped = 0;
+    s->skipped = 0;
         s->n_valid = 0;
         s->updates = 0;
         s->norm_delta = 0.0;
-        s->v_min = INT_MAX;
-        s->v_max = INT_MIN;
-        s->u_min = INT_MAX;
-        s->u_max = INT_MIN;
-        s->nv32 = 0;
-        s->n_buckets = 0;
-        s->total_items = 0;
-        s->flags = 0;
-        for (int q = 0; q < kXcdGroups; ++q) s->item_next[q] = 0;
-        s->nm32 = 0;
-        s->run_updates = 0;
-    }
-    for (int i = t; i < n_combo; i += gridDim.x * blockDim.x) combo[i] = 0u;
+    s->v_min = INT_MAX;
+    s->v_max = INT_MIN;
+    s->u_min = INT_MAX;
+    s->u_max = INT_MIN;
+    s->nv32 = 0;
+    s->n_buckets = 0;
+    s->total_items = 0;
+    s->flags = 0;
+    for (int q = 0; q < kXcdGroups; ++q) s->item_next[q] = 0;
+    s->nm32 = 0;
+    s->run_updates = 0;
+}
+
+// The summaries s[0..ns) reset and combo[0..n_combo) cleared (one call's
+// summary and histogram, or every job's of a batch at once).
+__global__ void k_init(Summary* __restrict__ s, unsigned* __restrict__ combo,
+                       size_t n_combo, int ns)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (size_t)ns) init_summary(s + t);
+    for (size_t i = t; i < n_combo; i += (size_t)gridDim.x * blockDim.x)
+        combo[i] = 0u;
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n)
@@ -560,7 +565,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(
 // -------------------------------------------------------------------------
 
 // Points gridded, update count and norm increment.  One block of 1024.
-__global__ void __launch_bounds__(1024) k_plan(
+__device__ __forceinline__ void plan_body(
         int P, int K2, const unsigned* __restrict__ combo,
         const PlaneInfo* __restrict__ pinfo,
         const double* __restrict__ normtab, Summary* __restrict__ summary)
@@ -609,6 +614,24 @@ __global__ void __launch_bounds__(1024) k_plan(
         summary->updates = upd_total;
         summary->norm_delta = s_d[0];
     }
+}
+
+__global__ void __launch_bounds__(1024) k_plan(
+        int P, int K2, const unsigned* __restrict__ combo,
+        const PlaneInfo* __restrict__ pinfo,
+        const double* __restrict__ normtab, Summary* __restrict__ summary)
+{
+    plan_body(P, K2, combo, pinfo, normtab, summary);
+}
+
+// k_plan for every job of a batch, one workgroup per job.
+__global__ void __launch_bounds__(1024) k_plan_batch(
+        const BatchJob* __restrict__ jobs,
+        const unsigned* __restrict__ combo, Summary* __restrict__ sums)
+{
+    const BatchJob& d = jobs[blockIdx.x];
+    plan_body(d.planes, d.sub_k * d.sub_k, combo + d.combo_off, d.pinfo,
+              d.normtab, sums + blockIdx.x);
 }
 
 // Sorted point records.  meta = (u0 | v0 << 16, sub-cell table
@@ -2616,7 +2639,7 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
     const int n_combo = (int)(P_ * K_ * K_);
     hipLaunchKernelGGL(k_init, dim3(std::max(1, std::min(1024,
             (n_combo + 255) / 256))), dim3(256), 0, stream_, d_summary_,
-            d_combo_, n_combo);
+            d_combo_, (size_t)n_combo, 1);
     const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
             (n + kLocateThreads - 1) / kLocateThreads, (size_t)n_cu_));
     const T gscale = sizeof(T) == 4 ? (T)gscale_f_ : (T)gscale_d_;
@@ -3447,6 +3470,9 @@ void Batch::prepare()
         d.G = pl.grid_size();
         d.R = pl.region();
         d.tile = pl.tile();
+        d.planes = (int)pl.num_w_planes();
+        d.normtab = pl.device_normtab();
+        d.combo_off = (unsigned)combos;
         d.tile_bits = pl.tile_bits();
         d.tile_shift = pl.tile_shift();
         d.key_bits = pl.key_bits();
@@ -3562,6 +3588,12 @@ void Batch::enqueue()
 {
     const hipStream_t s = stream_;
     const size_t J = jobs_.size();
+    // Every job's summary and (plane, sub-cell) histogram cleared at once.
+    const size_t combos = J ? combo_off_[J] : 0;
+    if (J)
+        hipLaunchKernelGGL(k_init, dim3((unsigned)std::max<size_t>(1,
+                std::min<size_t>(1024, (std::max(combos, J) + 255) / 256))),
+                dim3(256), 0, s, d_sum_, d_combo_, combos, (int)J);
     for (size_t j = 0; j < J; ++j)
     {
         const Entry& e = jobs_[j];
@@ -3569,11 +3601,7 @@ void Batch::enqueue()
         const BatchJob& d = desc_[j];
         const int P = (int)pl.num_w_planes();
         const int K = pl.sub_tables();
-        const int n_combo = P * K * K;
         unsigned* combo = d_combo_ + combo_off_[j];
-        hipLaunchKernelGGL(k_init, dim3(std::max(1, std::min(1024,
-                (n_combo + 255) / 256))), dim3(256), 0, s, d_sum_ + j, combo,
-                n_combo);
         if (e.n)
         {
             const int blocks = (int)std::max<size_t>(1, std::min<size_t>(
@@ -3611,10 +3639,12 @@ void Batch::enqueue()
                         (unsigned long long)j << kBatchJobShift,
                         d.point_base);
         }
-        hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, P, K * K,
-                (const unsigned*)combo, pl.device_pinfo(),
-                pl.device_normtab(), d_sum_ + j);
     }
+    // Every job's points gridded, updates and norm increment, one
+    // workgroup per job.
+    if (J)
+        hipLaunchKernelGGL(k_plan_batch, dim3((unsigned)J), dim3(1024), 0, s,
+                (const BatchJob*)d_desc_, (const unsigned*)d_combo_, d_sum_);
     check_hip(hipGetLastError(), kErrKernelLaunch, "batch index launch");
     const int pblocks = (int)std::max<size_t>(1, std::min<size_t>(
             (n_total_ + 255) / 256, (size_t)n_cu_ * 8));

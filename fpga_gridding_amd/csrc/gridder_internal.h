@@ -417,6 +417,9 @@ struct BatchJob
     unsigned n;
     int pos_shift;            // position bits start here in the job's keys
     int tile;                 // tile side
+    int planes;               // w-planes of the job's plan
+    const double* normtab;    // its per-(plane, sub-cell) tap sums
+    unsigned combo_off;       // its histogram in the batch's combo array
 };
 
 // Many independent FP32 gridding jobs (each with its own plan: kernel set,
