@@ -265,13 +265,12 @@ __global__ void __launch_bounds__(kLocateThreads) k_locate(
         T w_scale, int ov, int G, int P, const int* __restrict__ support,
         const int* __restrict__ srank, int srank_bits, int plane_bits,
         int h, int K, int tile, int tile_bits, int combo_bits, int key_bits,
-        unsigned long long* __restrict__ keys, unsigned* __restrict__ idx,
-        Rec* __restrict__ rec,
+        unsigned long long* __restrict__ keys, Rec* __restrict__ rec,
         unsigned* __restrict__ combo, Summary* __restrict__ summary,
         unsigned* __restrict__ gw_out,
         unsigned long long* __restrict__ diag_support,
         unsigned long long* __restrict__ diag_plane, int diag_plane_cap,
-        unsigned long long key_tag, unsigned idx_base)
+        unsigned long long key_tag)
 {
 #pragma clang fp contract(off)
     __shared__ unsigned s_combo[kComboLds];
@@ -375,7 +374,6 @@ __global__ void __launch_bounds__(kLocateThreads) k_locate(
             rec[i] = r;
         }
         keys[i] = key | key_tag;
-        idx[i] = idx_base + (unsigned)i;
     }
 
     // Block reductions: one atomic per block for the counters and box.
@@ -2651,9 +2649,9 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
                 d_support_, (const int*)d_srank_, srank_bits_, plane_bits_,
                 h_, K_,
                 tile_, tile_bits_, combo_bits_, key_bits_,
-                d_keys_[0], d_idx_[0],
+                d_keys_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, d_gw_, diag->per_support,
-                diag->per_plane, diag->plane_cap, 0ull, 0u);
+                diag->per_plane, diag->plane_cap, 0ull);
     else
         hipLaunchKernelGGL((k_locate<T, false, TU, TV, Rec>), dim3(blocks),
                 dim3(kLocateThreads), 0,
@@ -2661,10 +2659,10 @@ void Plan::locate_t(size_t n, const TU* uu, const TU* vv, const TU* ww,
                 d_support_, (const int*)d_srank_, srank_bits_, plane_bits_,
                 h_, K_,
                 tile_, tile_bits_, combo_bits_, key_bits_,
-                d_keys_[0], d_idx_[0],
+                d_keys_[0],
                 (Rec*)d_rec_, d_combo_, d_summary_, (unsigned*)nullptr,
                 (unsigned long long*)nullptr, (unsigned long long*)nullptr,
-                0, 0ull, 0u);
+                0, 0ull);
     check_hip(hipGetLastError(), kErrKernelLaunch, "k_locate launch");
     check_hip(hipEventRecord(ev_[1], stream_), kErrKernelLaunch, "event");
 }
@@ -2983,7 +2981,7 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_scatter(size_t n,
     {
         const size_t i = base + (size_t)j * 64;
         k[j] = i < n ? keys[i] : 0ull;
-        v[j] = i < n ? vals[i] : 0u;
+        v[j] = i < n ? (vals ? vals[i] : (unsigned)i) : 0u;
     }
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -3132,9 +3130,11 @@ int radix_sort_enqueue(size_t n, int bits, unsigned long long* keys[2],
                 hist);
         hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(kScanThreads), 0, s,
                 (const unsigned*)hist, nchunk, offs, tot);
+        // The first pass reads no values: they start as 0..n-1.
         hipLaunchKernelGGL(k_radix_scatter, dim3(nchunk),
                 dim3(kRadixThreads), 0, s, n,
                 (const unsigned long long*)keys[cur],
+                shift == 0 ? (const unsigned*)nullptr :
                 (const unsigned*)vals[cur], shift, nchunk,
                 (const unsigned*)offs, (const unsigned*)tot, keys[cur ^ 1],
                 vals[cur ^ 1]);
@@ -3616,13 +3616,12 @@ void Batch::enqueue()
                         pl.device_srank(), pl.srank_bits(), pl.plane_bits(),
                         d.sub_h,
                         K, d.tile, d.tile_bits, pl.combo_bits(), d.key_bits,
-                        d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
+                        d_keys_[0] + d.point_base,
                         static_cast<RecF*>(d_rec_) + d.point_base, combo,
                         d_sum_ + j, (unsigned*)nullptr,
                         (unsigned long long*)nullptr,
                         (unsigned long long*)nullptr, 0,
-                        (unsigned long long)j << kBatchJobShift,
-                        d.point_base);
+                        (unsigned long long)j << kBatchJobShift);
             else
                 hipLaunchKernelGGL((k_locate<float, false, float, float, RecF>),
                         dim3(blocks), dim3(kLocateThreads), 0, s, e.n, e.uu, e.vv, e.ww,
@@ -3631,13 +3630,12 @@ void Batch::enqueue()
                         pl.device_srank(), pl.srank_bits(), pl.plane_bits(),
                         d.sub_h,
                         K, d.tile, d.tile_bits, pl.combo_bits(), d.key_bits,
-                        d_keys_[0] + d.point_base, d_idx_[0] + d.point_base,
+                        d_keys_[0] + d.point_base,
                         static_cast<RecF*>(d_rec_) + d.point_base, combo,
                         d_sum_ + j, (unsigned*)nullptr,
                         (unsigned long long*)nullptr,
                         (unsigned long long*)nullptr, 0,
-                        (unsigned long long)j << kBatchJobShift,
-                        d.point_base);
+                        (unsigned long long)j << kBatchJobShift);
         }
     }
     // Every job's points gridded, updates and norm increment, one

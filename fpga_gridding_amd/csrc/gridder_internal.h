@@ -527,7 +527,8 @@ private:
 
 // Stable LSD radix sort of n (key, value) pairs on the low `bits` bits of
 // the keys (8-bit digits, one pass per digit: histogram, per-digit scan,
-// scatter), capture-safe.  keys[0] / vals[0] hold the input, keys[1] /
+// scatter), capture-safe.  keys[0] holds the input; the values start as
+// 0..n-1 (each key's input position; vals[0] is not read); keys[1] /
 // vals[1] are scratch of the same size; returns which pair holds the
 // result.  hist and offs hold radix_hist_words(n) words, tot 256 words.
 size_t radix_hist_words(size_t n);

@@ -58,7 +58,7 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads) k_weights_key(size_t n,
         const T* __restrict__ uu, const T* __restrict__ vv, T scale, int G,
         unsigned long long sentinel, unsigned long long* __restrict__ keys,
-        unsigned* __restrict__ idx, unsigned long long* __restrict__ skipped)
+        unsigned long long* __restrict__ skipped)
 {
     unsigned count = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -67,7 +67,6 @@ __global__ void __launch_bounds__(kThreads) k_weights_key(size_t n,
         const long long q = weights_cell(uu[i], vv[i], scale, G);
         count += q < 0;
         keys[i] = q < 0 ? sentinel : (unsigned long long)q;
-        idx[i] = (unsigned)i;
     }
     add_skipped(count, skipped);
 }
@@ -229,8 +228,7 @@ size_t Weights::write_t(size_t n, const T* uu, const T* vv, const T* weight,
     while ((cells >> bits) != 0) ++bits;
     const int blocks = blocks_for(n, n_cu_);
     hipLaunchKernelGGL(k_weights_key<T>, dim3(blocks), dim3(kThreads), 0,
-            st, n, uu, vv, scale, G, cells, d_keys_[0], d_idx_[0],
-            d_small_);
+            st, n, uu, vv, scale, G, cells, d_keys_[0], d_small_);
     const int cur = radix_sort_enqueue(n, bits, d_keys_, d_idx_, d_hist_,
             d_offs_, d_part_, st);
     hipLaunchKernelGGL(k_weights_sum<T>, dim3(blocks), dim3(kThreads), 0,
